@@ -1,0 +1,192 @@
+"""Throughput of the physics step on one MI355X.
+
+Workload (default): 1,048,576 spheres, 1024 x 1024 cells of size 1, half map
+height 8, radius 0.5, dt 1/60, e_agent = e_envir = 0.8, spring off, seeded
+uniform positions inside the box and velocities U[-1,1]^3 (per-step units).
+A "step" is one LOOP frame (radix sort -> spatial hashing -> collide +
+integrate) of that scene with its state resident in HBM; the INIT frame and
+the warmup frames run before the timed region.
+
+Prints one JSON line (rank 0).  `roofline` is computed for the collide +
+integrate kernel from its HIP-event durations over a profiled run of the same
+K frames; `cpu_baseline` times the oracle (the C transcription, OpenMP) on a
+bounded sample of the same workload on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from dataclasses import replace
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "sphere-steps/sec at 1M spheres; p50/p99 step latency, mixed-scene load"
+HBM_PEAK_GBS = 8000.0
+
+
+def physics_bytes_per_sphere(n: int, cells: int) -> float:
+    """Algorithmic HBM bytes of one collide + integrate launch per sphere:
+    perm 4 + state in 32 + cell ranges 8 * C / N + state out 32 + key out 4."""
+    return 72.0 + 8.0 * cells / n
+
+
+def sort_bytes_per_sphere() -> float:
+    """Compulsory bytes of the 3-pass 8-bit stable sort with generated values:
+    4 (digit counts) + 12 + 16 + 16 per key."""
+    return 48.0
+
+
+def cpu_baseline(cfg, pos, vf, budget_s: float, threads: int) -> dict:
+    from oracle import mb_oracle
+
+    w = mb_oracle.World(cfg.settings())
+    w.upload(pos, vf)
+    w.step(1, threads=threads)  # INIT frame, not counted
+    frames = 0
+    t0 = time.perf_counter()
+    while True:
+        w.step(1, threads=threads)
+        frames += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or frames >= 200:
+            break
+    w.close()
+    return {
+        "value": cfg.number_of_agents * frames / el,
+        "unit": "sphere-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{cfg.name}: {cfg.number_of_agents} spheres, {frames} LOOP frames after INIT, "
+                  f"{el:.1f} s, C transcription with OpenMP",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="uniform_1m")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--latency-steps", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(independent scenes shard by scene, one process per GPU).", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from bevy_millions_ball_amd import CONFIGS, Physics, generate
+
+    cfg = CONFIGS[args.config]
+    n = cfg.number_of_agents
+    cells = cfg.grids * cfg.grids
+    pos, vf = generate(cfg, seed=args.seed)
+    torch.cuda.set_device(0)
+    phys = Physics(cfg.settings())
+    phys.upload_state(pos, vf)
+    phys.run(1)  # INIT frame
+    phys.run(args.warmup)
+    torch.cuda.synchronize()
+
+    # -- throughput: K frames back to back -----------------------------------
+    t0 = time.perf_counter()
+    phys.run(args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = n * args.steps / elapsed
+
+    # -- per-kernel-family device time over K profiled frames ---------------
+    phys.set_profiling(True)
+    phys.run(args.steps)
+    torch.cuda.synchronize()
+    prof = phys.profile_read()
+    phys.set_profiling(False)
+    stage_ms = prof.mean(axis=0)
+    device_step_ms = prof.sum(axis=1)
+
+    # -- latency: each frame launched and waited for on its own ---------------
+    lat = []
+    for _ in range(args.latency_steps):
+        t = time.perf_counter()
+        phys.run(1)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t) * 1e3)
+    lat = np.array(lat)
+    phys.close()
+
+    phys_bytes = physics_bytes_per_sphere(n, cells) * n
+    achieved = phys_bytes / (stage_ms[2] * 1e-3) / 1e9
+    sort_gbs = sort_bytes_per_sphere() * n / (stage_ms[0] * 1e-3) / 1e9
+
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{cfg.name}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("physics_loop_bytes_per_launch")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "sphere-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": cfg.name,
+            "spheres": n,
+            "grids": cfg.grids,
+            "grid_size": cfg.grid_size,
+            "half_map_height": cfg.half_map_height,
+            "radius": cfg.radius,
+            "time_delta": cfg.time_delta,
+            "e_agent": cfg.e_agent,
+            "e_envir": cfg.e_envir,
+            "spring": cfg.enable_spring,
+            "layout": cfg.layout,
+            "seed": args.seed,
+            "parallelism": "single GPU",
+        },
+        "p50_step_ms": float(np.percentile(lat, 50)),
+        "p99_step_ms": float(np.percentile(lat, 99)),
+        "device_step_ms_p50": float(np.percentile(device_step_ms, 50)),
+        "device_step_ms_p99": float(np.percentile(device_step_ms, 99)),
+        "stage_ms": {"sort": float(stage_ms[0]), "spatial_hashing": float(stage_ms[1]),
+                     "collide_integrate": float(stage_ms[2])},
+        "sort_achieved_GBs": sort_gbs,
+        "roofline": {
+            "kernel": "physics_loop_kernel (collide + integrate)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "bytes_per_launch": phys_bytes,
+        },
+    }
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        line["cpu_baseline"] = cpu_baseline(cfg, pos, vf, args.cpu_seconds, threads)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,113 @@
+"""ctypes binding of libmillions_ball.so (include/millions_ball.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback behind this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libmillions_ball.so"
+
+MB_OK = 0
+MB_ERR_INVALID_ARGUMENT = -1
+MB_ERR_HIP = -2
+MB_ERR_OUT_OF_MEMORY = -3
+MB_ERR_STATE = -4
+MB_INVALID_INDEX = 0xFFFFFFFF
+
+
+class MbSettings(C.Structure):
+    """``mb_settings``: field for field the reference's PhysicsPlugin."""
+
+    _fields_ = [
+        ("enable_simulation", C.c_int32),
+        ("enable_spring", C.c_int32),
+        ("number_of_grids_one_dimension", C.c_uint32),
+        ("grid_size", C.c_uint32),
+        ("half_map_height", C.c_uint32),
+        ("gravity", C.c_float * 3),
+        ("e_envir", C.c_float),
+        ("e_agent", C.c_float),
+        ("k_agent", C.c_float),
+        ("c_agent", C.c_float),
+        ("r_agent", C.c_float),
+        ("max_number_of_agents", C.c_uint32),
+        ("time_delta", C.c_float),
+    ]
+
+
+class MbBuffers(C.Structure):
+    _fields_ = [
+        ("position_with_scale", C.c_void_p * 2),
+        ("packed_vfme", C.c_void_p * 2),
+        ("global_keys", C.c_void_p * 2),
+        ("global_vals", C.c_void_p * 2),
+        ("number_of_agents", C.c_uint32),
+    ]
+
+
+class MbError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"millions_ball error {code}: {message}")
+        self.code = code
+
+
+_lib = None
+
+_P = C.c_void_p
+_U32P = C.POINTER(C.c_uint32)
+_F32P = C.POINTER(C.c_float)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "mb_settings_default": (None, [C.POINTER(MbSettings)]),
+    "mb_physics_create": (C.c_int, [C.POINTER(MbSettings), C.POINTER(_P)]),
+    "mb_physics_destroy": (C.c_int, [_P]),
+    "mb_physics_set_settings": (C.c_int, [_P, C.POINTER(MbSettings)]),
+    "mb_physics_get_settings": (C.c_int, [_P, C.POINTER(MbSettings)]),
+    "mb_physics_upload_state": (C.c_int, [_P, _P, _P, _P]),
+    "mb_physics_upload_state_device": (C.c_int, [_P, _P, _P, _P]),
+    "mb_physics_download_state": (C.c_int, [_P, _P, _P, _P]),
+    "mb_physics_step": (C.c_int, [_P, C.c_uint32, _P]),
+    "mb_physics_time": (C.c_int64, [_P]),
+    "mb_physics_is_even": (C.c_int, [_P]),
+    "mb_physics_buffers": (C.c_int, [_P, C.POINTER(MbBuffers)]),
+    "mb_physics_debug_download": (C.c_int, [_P, _P, _P, _P, _P, _P]),
+    "mb_physics_set_profiling": (C.c_int, [_P, C.c_int]),
+    "mb_physics_profile_read": (C.c_int, [_P, _P, C.c_uint32, _U32P]),
+    "mb_last_error": (C.c_char_p, []),
+    "mb_radix_sort_workspace_size": (C.c_size_t, [C.c_uint32]),
+    "mb_radix_sort": (
+        C.c_int,
+        [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P],
+    ),
+    "mb_spatial_hashing": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P]),
+    "mb_physics_run": (
+        C.c_int,
+        [C.POINTER(MbSettings), C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, C.c_uint32, _P],
+    ),
+}
+
+
+def lib() -> C.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m bevy_millions_ball_amd.build`"
+            )
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != MB_OK:
+        raise MbError(rc, lib().mb_last_error().decode(errors="replace"))

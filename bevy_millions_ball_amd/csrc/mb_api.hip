@@ -1,0 +1,508 @@
+// mb_api.hip -- the C ABI declared in include/millions_ball.h.
+//
+// Host side of the drop-in boundary: the PhysicsPlugin / PhysicsSettings
+// surface (src/physics/mod.rs:53-436), the ping-pong buffers and their roles
+// per frame (mod.rs:659-705), the `time` counter (mod.rs:711-743, 804-839) and
+// the three stage entry points (bevy_radix_sort::run, spatial_hashing::run,
+// physics::run).  Kernels live in mb_kernels.hip.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <array>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/millions_ball.h"
+#include "mb_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int fail_hip(hipError_t e, const char *what)
+{
+    return fail(MB_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define MB_HIP(call)                                    \
+    do {                                                \
+        hipError_t _e = (call);                         \
+        if (_e != hipSuccess) return fail_hip(_e, #call); \
+    } while (0)
+
+// Makes the handle's device current for the duration of a call.
+struct DeviceGuard {
+    int prev = -1;
+    bool changed = false;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) {
+            changed = hipSetDevice(dev) == hipSuccess;
+        }
+    }
+    ~DeviceGuard()
+    {
+        if (changed) (void)hipSetDevice(prev);
+    }
+};
+
+// Rust `f as i32` / `f as u32`: truncation toward zero, saturation, NaN -> 0.
+int32_t rust_as_i32(float f)
+{
+    if (f != f) return 0;
+    if (f <= -2147483648.0f) return INT32_MIN;
+    if (f >= 2147483648.0f) return INT32_MAX;
+    return (int32_t)f;
+}
+
+uint32_t rust_as_u32(float f)
+{
+    if (!(f > 0.0f)) return 0u;
+    if (f >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)f;
+}
+
+// CompiledConsts::shader_defs (mod.rs:459-485) followed by the divisions in
+// physics.wgsl:7-16: x -> f32(int(x * 1000.)) / 1000.
+float quantize_signed(float x) { return (float)rust_as_i32(x * 1000.0f) / 1000.0f; }
+float quantize_unsigned(float x) { return (float)rust_as_u32(x * 1000.0f) / 1000.0f; }
+
+mb::PhysConsts compile_consts(const mb_settings &s)
+{
+    mb::PhysConsts c;
+    c.grids = s.number_of_grids_one_dimension;
+    c.grid_size = s.grid_size;
+    c.half_map_height = (float)s.half_map_height;
+    // half_map_size_u = (G * grid_size) / 2 in u32 (mod.rs:389-399)
+    c.half_map_size = (float)((s.number_of_grids_one_dimension * s.grid_size) / 2u);
+    c.gravity_x = quantize_signed(s.gravity[0]);
+    c.gravity_y = quantize_signed(s.gravity[1]);
+    c.gravity_z = quantize_signed(s.gravity[2]);
+    c.e_envir = quantize_unsigned(s.e_envir);
+    c.e_agent = quantize_unsigned(s.e_agent);
+    c.k_agent = quantize_unsigned(s.k_agent);
+    c.c_agent = quantize_unsigned(s.c_agent);
+    c.r_agent = quantize_unsigned(s.r_agent);
+    c.t_delta = quantize_unsigned(s.time_delta);
+    c.neg_c_over_dt = -c.c_agent / c.t_delta;
+    c.neg_1pe_over_dt = -(1.0f + c.e_envir) / c.t_delta;
+    c.enable_spring = s.enable_spring ? 1 : 0;
+    return c;
+}
+
+// The constraints the reference documents for its buffers and sort
+// (mod.rs:305-320, 351-362) plus what keeps every index in range here.
+int validate_settings(const mb_settings &s)
+{
+    const uint64_t n = s.max_number_of_agents;
+    const uint64_t g = s.number_of_grids_one_dimension;
+    if (n == 0 || n >= (1ull << 30))
+        return fail(MB_ERR_INVALID_ARGUMENT, "max_number_of_agents must be in [1, 2^30)");
+    if (g == 0) return fail(MB_ERR_INVALID_ARGUMENT, "number_of_grids_one_dimension must be >= 1");
+    if (s.grid_size == 0) return fail(MB_ERR_INVALID_ARGUMENT, "grid_size must be >= 1");
+    if (g * g > (1ull << 24))
+        return fail(MB_ERR_INVALID_ARGUMENT,
+                    "number_of_grids_one_dimension^2 exceeds 2^24 (the 24-bit cell key)");
+    if (g * g > n)
+        return fail(MB_ERR_INVALID_ARGUMENT,
+                    "number_of_grids_one_dimension^2 exceeds max_number_of_agents");
+    if (g * (uint64_t)s.grid_size > 0xFFFFFFFFull)
+        return fail(MB_ERR_INVALID_ARGUMENT, "number_of_grids_one_dimension * grid_size overflows u32");
+    return MB_OK;
+}
+
+// 8-bit passes needed to sort keys below `cells`.
+uint32_t sort_passes_for(uint32_t cells)
+{
+    uint32_t bits = 0;
+    while (bits < 32 && (cells - 1u) >> bits) ++bits;
+    const uint32_t passes = (bits + 7u) / 8u;
+    return passes == 0 ? 1u : passes;
+}
+
+bool finite_half(uint32_t h16) { return ((h16 >> 10) & 0x1Fu) != 0x1Fu; }
+
+}  // namespace
+
+struct mb_physics {
+    mb_settings settings;
+    mb::PhysConsts consts;
+    uint32_t n = 0;
+    int device = 0;
+    int64_t time = -1;
+    float4 *pos[2] = {nullptr, nullptr};
+    uint4 *vfme[2] = {nullptr, nullptr};
+    uint32_t *keys[2] = {nullptr, nullptr};
+    uint32_t *vals[2] = {nullptr, nullptr};
+    void *sort_ws = nullptr;
+    // Profiling: four events per frame (before sort, before hashing, before
+    // physics, after physics) recorded without blocking; read back by
+    // mb_physics_profile_read.
+    bool profiling = false;
+    std::vector<std::array<hipEvent_t, 4>> prof_events;
+    size_t prof_used = 0;
+    std::vector<double> prof_ms;  // [frame][3] of frames already read back
+};
+
+extern "C" {
+
+void mb_settings_default(mb_settings *s)
+{
+    if (!s) return;
+    s->enable_simulation = 1;
+    s->enable_spring = 0;
+    s->number_of_grids_one_dimension = 128;
+    s->grid_size = 8;
+    s->half_map_height = 16;
+    s->gravity[0] = 0.0f;
+    s->gravity[1] = -9.8f;
+    s->gravity[2] = 0.0f;
+    s->e_envir = 0.90f;
+    s->e_agent = 0.95f;
+    s->k_agent = 25.0f;
+    s->c_agent = 0.1f;
+    s->r_agent = 0.9f;
+    s->max_number_of_agents = 16u * 1024u;
+    s->time_delta = 1.0f / 120.0f;
+}
+
+const char *mb_last_error(void) { return g_last_error.c_str(); }
+
+static void release(mb_physics *p)
+{
+    for (int i = 0; i < 2; ++i) {
+        if (p->pos[i]) (void)hipFree(p->pos[i]);
+        if (p->vfme[i]) (void)hipFree(p->vfme[i]);
+        if (p->keys[i]) (void)hipFree(p->keys[i]);
+        if (p->vals[i]) (void)hipFree(p->vals[i]);
+    }
+    if (p->sort_ws) (void)hipFree(p->sort_ws);
+    for (auto &fe : p->prof_events)
+        for (hipEvent_t e : fe)
+            if (e) (void)hipEventDestroy(e);
+    p->prof_events.clear();
+}
+
+// Moves the durations of every recorded frame into prof_ms.
+static int drain_profile(mb_physics *p)
+{
+    if (p->prof_used == 0) return MB_OK;
+    MB_HIP(hipEventSynchronize(p->prof_events[p->prof_used - 1][3]));
+    for (size_t i = 0; i < p->prof_used; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            float ms = 0.0f;
+            MB_HIP(hipEventElapsedTime(&ms, p->prof_events[i][k], p->prof_events[i][k + 1]));
+            p->prof_ms.push_back(ms);
+        }
+    }
+    p->prof_used = 0;
+    return MB_OK;
+}
+
+int mb_physics_create(const mb_settings *settings, mb_physics **out)
+{
+    if (!settings || !out) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    int rc = validate_settings(*settings);
+    if (rc != MB_OK) return rc;
+    mb_physics *p = new (std::nothrow) mb_physics();
+    if (!p) return fail(MB_ERR_OUT_OF_MEMORY, "host allocation failed");
+    p->settings = *settings;
+    p->consts = compile_consts(*settings);
+    p->n = settings->max_number_of_agents;
+    if (hipGetDevice(&p->device) != hipSuccess) {
+        delete p;
+        return fail(MB_ERR_HIP, "no HIP device");
+    }
+    const size_t n = p->n;
+    const size_t ws = mb::sort_workspace_layout(p->n).bytes;
+    bool ok = true;
+    for (int i = 0; i < 2 && ok; ++i) {
+        ok = hipMalloc(&p->pos[i], n * sizeof(float4)) == hipSuccess &&
+             hipMalloc(&p->vfme[i], n * sizeof(uint4)) == hipSuccess &&
+             hipMalloc(&p->keys[i], n * 4u) == hipSuccess &&
+             hipMalloc(&p->vals[i], n * 4u) == hipSuccess;
+    }
+    ok = ok && hipMalloc(&p->sort_ws, ws) == hipSuccess;
+    if (!ok) {
+        release(p);
+        delete p;
+        return fail(MB_ERR_OUT_OF_MEMORY, "device allocation failed");
+    }
+    // Buffers start zeroed, as wgpu's do (the INIT frame sorts all-zero keys).
+    ok = hipMemset(p->sort_ws, 0, ws) == hipSuccess;
+    for (int i = 0; i < 2 && ok; ++i) {
+        ok = hipMemset(p->pos[i], 0, n * sizeof(float4)) == hipSuccess &&
+             hipMemset(p->vfme[i], 0, n * sizeof(uint4)) == hipSuccess &&
+             hipMemset(p->keys[i], 0, n * 4u) == hipSuccess &&
+             hipMemset(p->vals[i], 0, n * 4u) == hipSuccess;
+    }
+    ok = ok && hipDeviceSynchronize() == hipSuccess;
+    if (!ok) {
+        release(p);
+        delete p;
+        return fail(MB_ERR_HIP, "device initialisation failed");
+    }
+    *out = p;
+    return MB_OK;
+}
+
+int mb_physics_destroy(mb_physics *p)
+{
+    if (!p) return MB_OK;
+    DeviceGuard g(p->device);
+    (void)hipDeviceSynchronize();
+    release(p);
+    delete p;
+    return MB_OK;
+}
+
+int mb_physics_get_settings(const mb_physics *p, mb_settings *s)
+{
+    if (!p || !s) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    *s = p->settings;
+    return MB_OK;
+}
+
+int mb_physics_set_settings(mb_physics *p, const mb_settings *s)
+{
+    if (!p || !s) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    if (s->max_number_of_agents != p->n)
+        return fail(MB_ERR_INVALID_ARGUMENT,
+                    "max_number_of_agents is fixed when the plugin is built");
+    int rc = validate_settings(*s);
+    if (rc != MB_OK) return rc;
+    const bool regrid = s->number_of_grids_one_dimension != p->settings.number_of_grids_one_dimension ||
+                        s->grid_size != p->settings.grid_size;
+    p->settings = *s;
+    p->consts = compile_consts(*s);
+    if (regrid && p->time >= 0) {
+        DeviceGuard g(p->device);
+        const int cur = (int)((p->time + 1) & 1);
+        hipError_t e = mb::launch_rekey(p->consts, p->pos[cur], p->keys[cur], p->n, nullptr);
+        if (e != hipSuccess) return fail_hip(e, "rekey");
+        MB_HIP(hipStreamSynchronize(nullptr));
+    }
+    return MB_OK;
+}
+
+int mb_physics_upload_state(mb_physics *p, const float *pos, const uint32_t *vfme, void *stream)
+{
+    if (!p || !pos || !vfme) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    for (size_t i = 0; i < (size_t)p->n * 4u; ++i) {
+        if (!std::isfinite(pos[i]))
+            return fail(MB_ERR_INVALID_ARGUMENT, "non-finite position_with_scale at float " + std::to_string(i));
+        if (!finite_half(vfme[i] & 0xFFFFu) || !finite_half(vfme[i] >> 16))
+            return fail(MB_ERR_INVALID_ARGUMENT, "non-finite half in packed_vfme at word " + std::to_string(i));
+    }
+    DeviceGuard g(p->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    MB_HIP(hipMemcpyAsync(p->pos[0], pos, (size_t)p->n * sizeof(float4), hipMemcpyHostToDevice, st));
+    MB_HIP(hipMemcpyAsync(p->vfme[0], vfme, (size_t)p->n * sizeof(uint4), hipMemcpyHostToDevice, st));
+    return mb_physics_upload_state_device(p, nullptr, nullptr, stream);
+}
+
+int mb_physics_upload_state_device(mb_physics *p, const void *pos, const void *vfme, void *stream)
+{
+    if (!p) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    DeviceGuard g(p->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t n = p->n;
+    if (pos) MB_HIP(hipMemcpyAsync(p->pos[0], pos, n * sizeof(float4), hipMemcpyDeviceToDevice, st));
+    if (vfme) MB_HIP(hipMemcpyAsync(p->vfme[0], vfme, n * sizeof(uint4), hipMemcpyDeviceToDevice, st));
+    MB_HIP(hipMemsetAsync(p->pos[1], 0, n * sizeof(float4), st));
+    MB_HIP(hipMemsetAsync(p->vfme[1], 0, n * sizeof(uint4), st));
+    for (int i = 0; i < 2; ++i) {
+        MB_HIP(hipMemsetAsync(p->keys[i], 0, n * 4u, st));
+        MB_HIP(hipMemsetAsync(p->vals[i], 0, n * 4u, st));
+    }
+    MB_HIP(hipStreamSynchronize(st));
+    p->time = -1;
+    return MB_OK;
+}
+
+int mb_physics_download_state(const mb_physics *p, float *pos, uint32_t *vfme, void *stream)
+{
+    if (!p) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    DeviceGuard g(p->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int cur = (int)((p->time + 1) & 1);
+    if (pos) MB_HIP(hipMemcpyAsync(pos, p->pos[cur], (size_t)p->n * sizeof(float4), hipMemcpyDeviceToHost, st));
+    if (vfme) MB_HIP(hipMemcpyAsync(vfme, p->vfme[cur], (size_t)p->n * sizeof(uint4), hipMemcpyDeviceToHost, st));
+    MB_HIP(hipStreamSynchronize(st));
+    return MB_OK;
+}
+
+static int run_frame(mb_physics *p, hipStream_t st)
+{
+    p->time += 1;
+    const int par = (int)(p->time & 1);
+    const int q = 1 - par;
+    const uint32_t cells = p->settings.number_of_grids_one_dimension * p->settings.number_of_grids_one_dimension;
+    std::array<hipEvent_t, 4> *ev = nullptr;
+    if (p->profiling) {
+        constexpr size_t kRing = 4096;
+        if (p->prof_used == kRing) {
+            int rc = drain_profile(p);
+            if (rc != MB_OK) return rc;
+        }
+        if (p->prof_used == p->prof_events.size()) {
+            std::array<hipEvent_t, 4> fe{};
+            for (hipEvent_t &e : fe) MB_HIP(hipEventCreate(&e));
+            p->prof_events.push_back(fe);
+        }
+        ev = &p->prof_events[p->prof_used++];
+        MB_HIP(hipEventRecord((*ev)[0], st));
+    }
+    // bevy_radix_sort::run(.., N, 0..3, true, is_even): keys[p] -> keys[q], vals[q]
+    hipError_t e = mb::launch_radix_sort(p->sort_ws, p->n, p->keys[par], nullptr, p->keys[q],
+                                         p->vals[q], p->n, 0u, sort_passes_for(cells), st);
+    if (e != hipSuccess) return fail_hip(e, "radix sort");
+    if (ev) MB_HIP(hipEventRecord((*ev)[1], st));
+    // spatial_hashing::run(.., N, !is_even): start -> keys[p], close -> vals[p]
+    e = mb::launch_spatial_hashing(p->keys[q], p->keys[par], p->vals[par], p->n, p->n, st);
+    if (e != hipSuccess) return fail_hip(e, "spatial hashing");
+    if (ev) MB_HIP(hipEventRecord((*ev)[2], st));
+    // physics::run: INIT at time 0, else LOOP (mod.rs:929-939)
+    e = mb::launch_physics(p->consts, p->time == 0, p->vals[q], p->keys[par], p->vals[par],
+                           p->keys[q], p->pos[par], p->vfme[par], p->pos[q], p->vfme[q], p->n, st);
+    if (e != hipSuccess) return fail_hip(e, "physics");
+    if (ev) MB_HIP(hipEventRecord((*ev)[3], st));
+    return MB_OK;
+}
+
+int mb_physics_step(mb_physics *p, uint32_t frames, void *stream)
+{
+    if (!p) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    if (!p->settings.enable_simulation) return MB_OK;
+    DeviceGuard g(p->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    for (uint32_t f = 0; f < frames; ++f) {
+        int rc = run_frame(p, st);
+        if (rc != MB_OK) return rc;
+    }
+    return MB_OK;
+}
+
+int64_t mb_physics_time(const mb_physics *p) { return p ? p->time : -1; }
+
+int mb_physics_is_even(const mb_physics *p)
+{
+    if (!p || p->time < 0) return -1;
+    return (p->time % 2 == 0) ? 1 : 0;
+}
+
+int mb_physics_buffers(const mb_physics *p, mb_buffers *out)
+{
+    if (!p || !out) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    for (int i = 0; i < 2; ++i) {
+        out->position_with_scale[i] = p->pos[i];
+        out->packed_vfme[i] = p->vfme[i];
+        out->global_keys[i] = p->keys[i];
+        out->global_vals[i] = p->vals[i];
+    }
+    out->number_of_agents = p->n;
+    return MB_OK;
+}
+
+int mb_physics_debug_download(const mb_physics *p, uint32_t *perm, uint32_t *start,
+                              uint32_t *close, uint32_t *next_keys, void *stream)
+{
+    if (!p) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    if (p->time < 0) return fail(MB_ERR_STATE, "no frame has run yet");
+    DeviceGuard g(p->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int par = (int)(p->time & 1);
+    const int q = 1 - par;
+    const size_t bytes = (size_t)p->n * 4u;
+    if (perm) MB_HIP(hipMemcpyAsync(perm, p->vals[q], bytes, hipMemcpyDeviceToHost, st));
+    if (start) MB_HIP(hipMemcpyAsync(start, p->keys[par], bytes, hipMemcpyDeviceToHost, st));
+    if (close) MB_HIP(hipMemcpyAsync(close, p->vals[par], bytes, hipMemcpyDeviceToHost, st));
+    if (next_keys) MB_HIP(hipMemcpyAsync(next_keys, p->keys[q], bytes, hipMemcpyDeviceToHost, st));
+    MB_HIP(hipStreamSynchronize(st));
+    return MB_OK;
+}
+
+int mb_physics_set_profiling(mb_physics *p, int enable)
+{
+    if (!p) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    p->profiling = enable != 0;
+    return MB_OK;
+}
+
+int mb_physics_profile_read(mb_physics *p, double *stage_ms, uint32_t capacity,
+                            uint32_t *frames)
+{
+    if (!p || !frames) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    DeviceGuard g(p->device);
+    int rc = drain_profile(p);
+    if (rc != MB_OK) return rc;
+    const size_t have = p->prof_ms.size() / 3;
+    const size_t take = have < capacity ? have : capacity;
+    if (stage_ms && take) std::memcpy(stage_ms, p->prof_ms.data(), take * 3 * sizeof(double));
+    *frames = (uint32_t)take;
+    p->prof_ms.clear();
+    return MB_OK;
+}
+
+size_t mb_radix_sort_workspace_size(uint32_t max_number_of_keys)
+{
+    return mb::sort_workspace_layout(max_number_of_keys).bytes;
+}
+
+int mb_radix_sort(void *workspace, const uint32_t *keys_in, const uint32_t *vals_in,
+                  uint32_t *keys_out, uint32_t *vals_out, uint32_t n, uint32_t pass_begin,
+                  uint32_t pass_end, void *stream)
+{
+    if (n == 0) return MB_OK;
+    if (!workspace || !keys_in || !keys_out || !vals_out)
+        return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    if (pass_end > 4u || pass_begin > pass_end)
+        return fail(MB_ERR_INVALID_ARGUMENT, "pass range must lie in 0..4");
+    if (pass_begin == pass_end && vals_in == nullptr)
+        return fail(MB_ERR_INVALID_ARGUMENT, "init_vals needs at least one pass");
+    if (keys_in == keys_out || (vals_in && vals_in == vals_out))
+        return fail(MB_ERR_INVALID_ARGUMENT, "input and output must not alias");
+    if (n >= (1u << 30)) return fail(MB_ERR_INVALID_ARGUMENT, "number_of_keys must be < 2^30");
+    hipError_t e = mb::launch_radix_sort(workspace, n, keys_in, vals_in, keys_out, vals_out, n,
+                                         pass_begin, pass_end, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail_hip(e, "radix sort");
+    return MB_OK;
+}
+
+int mb_spatial_hashing(const uint32_t *sorted_keys, uint32_t *start, uint32_t *close, uint32_t n,
+                       uint32_t zero_count, void *stream)
+{
+    if (!start || (n && (!sorted_keys || !close))) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    hipError_t e = mb::launch_spatial_hashing(sorted_keys, start, close, n, zero_count,
+                                              static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail_hip(e, "spatial hashing");
+    return MB_OK;
+}
+
+int mb_physics_run(const mb_settings *s, int init, const uint32_t *perm, const uint32_t *start,
+                   const uint32_t *close, uint32_t *key_out, const void *pos_i, const void *vfme_i,
+                   void *pos_o, void *vfme_o, uint32_t n, void *stream)
+{
+    if (!s) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    if (n == 0) return MB_OK;
+    if (!perm || !key_out || !pos_i || !vfme_i || !pos_o || !vfme_o || (!init && (!start || !close)))
+        return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    if (s->number_of_grids_one_dimension == 0 || s->grid_size == 0)
+        return fail(MB_ERR_INVALID_ARGUMENT, "grid count and grid size must be >= 1");
+    hipError_t e = mb::launch_physics(compile_consts(*s), init != 0, perm, start, close, key_out,
+                                      static_cast<const float4 *>(pos_i), static_cast<const uint4 *>(vfme_i),
+                                      static_cast<float4 *>(pos_o), static_cast<uint4 *>(vfme_o), n,
+                                      static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail_hip(e, "physics");
+    return MB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,760 @@
+// mb_kernels.hip -- CDNA4 (gfx950) kernels of the physics step.
+//
+// Stages (reference: src/physics/mod.rs:880-909):
+//   1. stable LSD radix sort of u32 cell keys with u32 indices
+//      (replaces bevy_radix_sort::run, mod.rs:880-890);
+//   2. spatial hashing: per-cell [start, close] of the sorted keys
+//      (spatial_hashing.wgsl:29-70);
+//   3. physics INIT / LOOP: collide + Verlet integrate + next cell key
+//      (physics.wgsl:57-97).
+//
+// Floating point follows physics.wgsl operation by operation: the library is
+// built with -ffp-contract=off, sqrtf and '/' are the correctly rounded HIP
+// defaults, min/max/clamp are compare-and-select, f32 -> f16 rounds to
+// nearest even.  Every stage is integer or elementwise work bound by HBM
+// bandwidth; nothing is a dense contraction, so there is no MFMA here.
+#include "mb_internal.h"
+
+namespace mb {
+
+// ---------------------------------------------------------------------------
+// Wave / block primitives (wave64)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t lanemask_lt(uint32_t lane)
+{
+    return (1ull << lane) - 1ull;
+}
+
+// Inclusive prefix sum across the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x, uint32_t lane)
+{
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+// Exclusive prefix sum over a 256-thread block; `total` receives the sum.
+// `scratch` holds 4 words of LDS.  Contains two barriers.
+__device__ __forceinline__ uint32_t block256_exclusive_scan(uint32_t v, uint32_t *scratch,
+                                                            uint32_t &total)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = threadIdx.x >> 6;
+    uint32_t inc = wave_inclusive_scan(v, lane);
+    if (lane == 63u) scratch[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0, sum = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) {
+        uint32_t t = scratch[w];
+        before += (w < wave) ? t : 0u;
+        sum += t;
+    }
+    total = sum;
+    __syncthreads();
+    return before + inc - v;
+}
+
+// Lanes of the wave whose 8-bit digit equals this lane's (ballot per bit).
+__device__ __forceinline__ uint64_t match_digit8(uint32_t d, bool valid)
+{
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (uint32_t b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+// ---------------------------------------------------------------------------
+// Utility kernels
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t *dst, uint32_t value,
+                                                       size_t count)
+{
+    size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * 256u;
+    for (; i < count; i += stride) dst[i] = value;
+}
+
+hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t count, hipStream_t stream)
+{
+    if (count == 0) return hipSuccess;
+    size_t blocks = (count + 255u) / 256u;
+    if (blocks > 8192u) blocks = 8192u;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((uint32_t)blocks), dim3(256), 0, stream, dst,
+                       value, count);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Radix sort: per pass  count -> column scan -> stable scatter
+// ---------------------------------------------------------------------------
+// Tile t owns keys [t * kSortTile, (t + 1) * kSortTile).  counts is digit
+// major: counts[d * tiles + t].  After the scan kernel it holds the global
+// output offset of digit d's first key in tile t, so the exclusive scan over
+// the flattened array gives every (digit, tile) its slot range and the
+// scatter needs no cross-tile communication.
+
+// Per-tile digit counts.  Each thread counts a contiguous run of kSortItems
+// keys and merges equal consecutive digits before touching LDS, so the high
+// passes over nearly sorted keys (long runs of one digit) cost one LDS atomic
+// per run instead of one per key.
+__global__ __launch_bounds__(kSortBlock) void radix_count_kernel(
+    const uint32_t *__restrict__ keys, uint32_t n, uint32_t shift, uint32_t tiles,
+    uint32_t *__restrict__ counts, uint32_t *__restrict__ hist)
+{
+    __shared__ uint32_t s_hist[kRadix];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t tile = blockIdx.x;
+    s_hist[tid] = 0u;
+    __syncthreads();
+
+    const uint32_t beg = tile * kSortTile + tid * kSortItems;
+    uint32_t cur = kInvalid, run = 0;
+    if (beg + kSortItems <= n && ((reinterpret_cast<uintptr_t>(keys + beg) & 15u) == 0)) {
+        const uint4 *v = reinterpret_cast<const uint4 *>(keys + beg);
+        uint4 q[kSortItems / 4];
+#pragma unroll
+        for (uint32_t j = 0; j < kSortItems / 4; ++j) q[j] = v[j];
+#pragma unroll
+        for (uint32_t j = 0; j < kSortItems / 4; ++j) {
+            const uint32_t kk[4] = {q[j].x, q[j].y, q[j].z, q[j].w};
+#pragma unroll
+            for (uint32_t e = 0; e < 4; ++e) {
+                const uint32_t d = (kk[e] >> shift) & (kRadix - 1u);
+                if (d == cur) {
+                    ++run;
+                } else {
+                    if (run) atomicAdd(&s_hist[cur], run);
+                    cur = d;
+                    run = 1u;
+                }
+            }
+        }
+    } else {
+        for (uint32_t j = 0; j < kSortItems; ++j) {
+            const uint32_t idx = beg + j;
+            if (idx >= n) break;
+            const uint32_t d = (keys[idx] >> shift) & (kRadix - 1u);
+            if (d == cur) {
+                ++run;
+            } else {
+                if (run) atomicAdd(&s_hist[cur], run);
+                cur = d;
+                run = 1u;
+            }
+        }
+    }
+    if (run) atomicAdd(&s_hist[cur], run);
+    __syncthreads();
+    const uint32_t c = s_hist[tid];
+    counts[(size_t)tid * tiles + tile] = c;
+    if (c) atomicAdd(&hist[tid], c);
+}
+
+// One block per digit: exclusive scan of the digit's column over tiles,
+// offset by the number of keys with a smaller digit.
+__global__ __launch_bounds__(kSortBlock) void radix_scan_kernel(
+    uint32_t *__restrict__ counts, const uint32_t *__restrict__ hist, uint32_t tiles)
+{
+    __shared__ uint32_t s_scratch[4];
+    const uint32_t d = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    uint32_t total;
+    const uint32_t hv = hist[tid];
+    const uint32_t before = block256_exclusive_scan(hv, s_scratch, total);
+    __shared__ uint32_t s_start;
+    if (tid == d) s_start = before;
+    __syncthreads();
+    uint32_t carry = s_start;
+    uint32_t *col = counts + (size_t)d * tiles;
+    for (uint32_t t0 = 0; t0 < tiles; t0 += kSortBlock) {
+        const uint32_t t = t0 + tid;
+        const uint32_t c = t < tiles ? col[t] : 0u;
+        uint32_t sum;
+        const uint32_t ex = block256_exclusive_scan(c, s_scratch, sum);
+        if (t < tiles) col[t] = carry + ex;
+        carry += sum;
+    }
+}
+
+// Stable scatter of one tile.  Keys are ranked in tile order (wave-major,
+// then 64-key rounds, then lane) with 8 ballots per round; the tile is then
+// reordered by digit in LDS so that the global writes are runs of equal
+// digits.  FIRST generates the identity values (init_vals = true).
+template <bool FIRST>
+__global__ __launch_bounds__(kSortBlock) void radix_scatter_kernel(
+    const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ vals_in,
+    uint32_t *__restrict__ keys_out, uint32_t *__restrict__ vals_out, uint32_t n,
+    uint32_t shift, uint32_t tiles, const uint32_t *__restrict__ offsets,
+    uint32_t *__restrict__ hist_to_clear)
+{
+    constexpr uint32_t kWaves = kSortBlock / 64u;
+    __shared__ uint32_t s_wave_hist[kWaves][kRadix];
+    __shared__ uint32_t s_local[kRadix];  // tile-local exclusive digit offset
+    __shared__ uint32_t s_base[kRadix];   // global offset - tile-local offset
+    __shared__ uint32_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint32_t s_scratch[4];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = tid >> 6;
+    const uint32_t tile = blockIdx.x;
+
+#pragma unroll
+    for (uint32_t w = 0; w < kWaves; ++w) s_wave_hist[w][tid] = 0u;
+    const uint32_t goff = offsets[(size_t)tid * tiles + tile];
+
+    const uint32_t base = tile * kSortTile + wave * (kSortItems * 64u);
+    uint32_t key[kSortItems];
+    uint32_t val[kSortItems];
+#pragma unroll
+    for (uint32_t j = 0; j < kSortItems; ++j) {
+        const uint32_t idx = base + j * 64u + lane;
+        const bool ok = idx < n;
+        key[j] = ok ? keys_in[idx] : 0u;
+        if (FIRST)
+            val[j] = idx;
+        else
+            val[j] = ok ? vals_in[idx] : 0u;
+    }
+    __syncthreads();
+
+    uint32_t rank[kSortItems];
+    const uint64_t lt = lanemask_lt(lane);
+#pragma unroll
+    for (uint32_t j = 0; j < kSortItems; ++j) {
+        const uint32_t idx = base + j * 64u + lane;
+        const bool ok = idx < n;
+        const uint32_t d = (key[j] >> shift) & (kRadix - 1u);
+        const uint64_t peers = match_digit8(d, ok);
+        uint32_t r = 0;
+        if (ok) {
+            const uint32_t before = s_wave_hist[wave][d];
+            const uint32_t pos = (uint32_t)__popcll(peers & lt);
+            r = before + pos;
+            if (pos == 0u) s_wave_hist[wave][d] = before + (uint32_t)__popcll(peers);
+        }
+        rank[j] = r;
+    }
+    __syncthreads();
+
+    // Digit tid: exclusive offsets across waves and the tile total.
+    uint32_t run = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kWaves; ++w) {
+        const uint32_t c = s_wave_hist[w][tid];
+        s_wave_hist[w][tid] = run;
+        run += c;
+    }
+    uint32_t tile_total;
+    const uint32_t local = block256_exclusive_scan(run, s_scratch, tile_total);
+    s_local[tid] = local;
+    s_base[tid] = goff - local;
+    __syncthreads();
+
+#pragma unroll
+    for (uint32_t j = 0; j < kSortItems; ++j) {
+        const uint32_t idx = base + j * 64u + lane;
+        if (idx < n) {
+            const uint32_t d = (key[j] >> shift) & (kRadix - 1u);
+            const uint32_t p = s_local[d] + s_wave_hist[wave][d] + rank[j];
+            s_keys[p] = key[j];
+            s_vals[p] = val[j];
+        }
+    }
+    __syncthreads();
+
+    const uint32_t tile_n = min(kSortTile, n - tile * kSortTile);
+    for (uint32_t i = tid; i < tile_n; i += kSortBlock) {
+        const uint32_t k = s_keys[i];
+        const uint32_t g = s_base[(k >> shift) & (kRadix - 1u)] + i;
+        keys_out[g] = k;
+        vals_out[g] = s_vals[i];
+    }
+    // The scan kernel has consumed this pass's digit totals.
+    if (tile == 0) hist_to_clear[tid] = 0u;
+}
+
+SortWorkspaceLayout sort_workspace_layout(uint32_t max_keys)
+{
+    SortWorkspaceLayout l;
+    const uint32_t tiles = max_keys == 0 ? 1u : (max_keys + kSortTile - 1u) / kSortTile;
+    auto align = [](size_t x) { return (x + 255u) & ~(size_t)255u; };
+    l.tiles = tiles;
+    l.hist_offset = 0;
+    l.counts_offset = align(kRadix * 4u);
+    l.keys_offset = align(l.counts_offset + (size_t)kRadix * tiles * 4u);
+    l.vals_offset = align(l.keys_offset + (size_t)max_keys * 4u);
+    l.bytes = align(l.vals_offset + (size_t)max_keys * 4u);
+    return l;
+}
+
+hipError_t launch_radix_sort(void *workspace, uint32_t max_keys, const uint32_t *keys_in,
+                             const uint32_t *vals_in, uint32_t *keys_out,
+                             uint32_t *vals_out, uint32_t n, uint32_t pass_begin,
+                             uint32_t pass_end, hipStream_t stream)
+{
+    if (n == 0) return hipSuccess;
+    const SortWorkspaceLayout l = sort_workspace_layout(max_keys);
+    char *ws = static_cast<char *>(workspace);
+    uint32_t *hist = reinterpret_cast<uint32_t *>(ws + l.hist_offset);
+    uint32_t *counts = reinterpret_cast<uint32_t *>(ws + l.counts_offset);
+    uint32_t *tmp_k = reinterpret_cast<uint32_t *>(ws + l.keys_offset);
+    uint32_t *tmp_v = reinterpret_cast<uint32_t *>(ws + l.vals_offset);
+    const uint32_t tiles = (n + kSortTile - 1u) / kSortTile;
+    const uint32_t passes = pass_end > pass_begin ? pass_end - pass_begin : 0u;
+
+    if (passes == 0) {
+        // No digit to sort by: the output is the input (the caller rejects
+        // vals_in == nullptr here, there being no pass to generate them in).
+        if (vals_in == nullptr) return hipErrorInvalidValue;
+        hipError_t e = hipMemcpyAsync(keys_out, keys_in, (size_t)n * 4u,
+                                      hipMemcpyDeviceToDevice, stream);
+        if (e != hipSuccess) return e;
+        return hipMemcpyAsync(vals_out, vals_in, (size_t)n * 4u, hipMemcpyDeviceToDevice,
+                              stream);
+    }
+
+    const uint32_t *src_k = keys_in;
+    const uint32_t *src_v = vals_in;
+    for (uint32_t i = 0; i < passes; ++i) {
+        const uint32_t shift = 8u * (pass_begin + i);
+        const bool to_out = ((passes - 1u - i) & 1u) == 0u;
+        uint32_t *dst_k = to_out ? keys_out : tmp_k;
+        uint32_t *dst_v = to_out ? vals_out : tmp_v;
+        hipLaunchKernelGGL(radix_count_kernel, dim3(tiles), dim3(kSortBlock), 0, stream, src_k,
+                           n, shift, tiles, counts, hist);
+        hipLaunchKernelGGL(radix_scan_kernel, dim3(kRadix), dim3(kSortBlock), 0, stream, counts,
+                           hist, tiles);
+        if (i == 0 && src_v == nullptr) {
+            hipLaunchKernelGGL(radix_scatter_kernel<true>, dim3(tiles), dim3(kSortBlock), 0,
+                               stream, src_k, src_v, dst_k, dst_v, n, shift, tiles, counts, hist);
+        } else {
+            hipLaunchKernelGGL(radix_scatter_kernel<false>, dim3(tiles), dim3(kSortBlock), 0,
+                               stream, src_k, src_v, dst_k, dst_v, n, shift, tiles, counts, hist);
+        }
+        src_k = dst_k;
+        src_v = dst_v;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Spatial hashing (spatial_hashing.wgsl:29-70)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void sh_hashing_kernel(const uint32_t *__restrict__ keys,
+                                                         uint32_t *__restrict__ start,
+                                                         uint32_t *__restrict__ close,
+                                                         uint32_t n, uint32_t cells)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = keys[i];
+    const uint32_t prv = i > 0u ? keys[i - 1u] : 0xFFFFFFFFu - k;
+    const uint32_t nxt = i < n - 1u ? keys[i + 1u] : 0xFFFFFFFFu - k;
+    if (k < cells) {
+        if (k != prv) start[k] = i;
+        if (k != nxt) close[k] = i;
+    }
+}
+
+hipError_t launch_spatial_hashing(const uint32_t *sorted_keys, uint32_t *start,
+                                  uint32_t *close, uint32_t n, uint32_t zero_count,
+                                  hipStream_t stream)
+{
+    hipError_t e = launch_fill_u32(start, kInvalid, zero_count, stream);
+    if (e != hipSuccess || n == 0) return e;
+    hipLaunchKernelGGL(sh_hashing_kernel, dim3((n + 255u) / 256u), dim3(256), 0, stream,
+                       sorted_keys, start, close, n, zero_count);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Physics (physics.wgsl)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float fmax_sel(float a, float b)
+{
+    if (a != a) return b;
+    if (b != b) return a;
+    return a > b ? a : b;
+}
+
+__device__ __forceinline__ float fmin_sel(float a, float b)
+{
+    if (a != a) return b;
+    if (b != b) return a;
+    return a < b ? a : b;
+}
+
+__device__ __forceinline__ float clamp_sel(float x, float lo, float hi)
+{
+    return fmin_sel(fmax_sel(x, lo), hi);
+}
+
+__device__ __forceinline__ float sign_sel(float x)
+{
+    return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f);
+}
+
+// WGSL u32(f32): truncation toward zero, saturating, NaN -> 0.
+__device__ __forceinline__ uint32_t f32_to_u32_sat(float x)
+{
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
+__device__ __forceinline__ float half_lo(uint32_t w)
+{
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w & 0xFFFFu));
+}
+
+__device__ __forceinline__ float half_hi(uint32_t w)
+{
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(w >> 16));
+}
+
+// pack2x16float with round-to-nearest-even conversions.
+__device__ __forceinline__ uint32_t pack_half2(float lo, float hi)
+{
+    const uint16_t a = __builtin_bit_cast(uint16_t, (_Float16)lo);
+    const uint16_t b = __builtin_bit_cast(uint16_t, (_Float16)hi);
+    return (uint32_t)a | ((uint32_t)b << 16);
+}
+
+// get_collision_grid_id (physics.wgsl:107-109), clamped to the grid.
+__device__ __forceinline__ void cell_of(const PhysConsts &c, float x, float z, uint32_t &cx,
+                                        uint32_t &cz)
+{
+    const uint32_t ix = f32_to_u32_sat(x + c.half_map_size) / c.grid_size;
+    const uint32_t iz = f32_to_u32_sat(z + c.half_map_size) / c.grid_size;
+    cx = min(ix, c.grids - 1u);
+    cz = min(iz, c.grids - 1u);
+}
+
+struct Agent {
+    float px, py, pz, scale;
+    float vx, vy, vz, m;
+    float fx, fy, fz;
+};
+
+__device__ __forceinline__ Agent agent_from(float4 p, uint4 q)
+{
+    Agent a;
+    a.px = p.x;
+    a.py = p.y;
+    a.pz = p.z;
+    a.scale = p.w;
+    a.vx = half_lo(q.x);
+    a.vy = half_hi(q.x);
+    a.vz = half_lo(q.y);
+    a.fx = half_hi(q.y);
+    a.fy = half_lo(q.z);
+    a.fz = half_hi(q.z);
+    a.m = half_lo(q.w);
+    return a;
+}
+
+// solve_sphere_constraint (physics.wgsl:300-331) against the neighbour in
+// sorted slot i.  The neighbour's packed state is read only when it is needed
+// (contact, rigid response).
+template <bool SPRING>
+__device__ __forceinline__ void solve_sphere(const PhysConsts &c, Agent &a, float4 bp,
+                                             const uint4 *__restrict__ vfme_i, uint32_t bo)
+{
+    const float edge = bp.w + a.scale;
+    const float tx = a.px - bp.x;
+    const float ty = a.py - bp.y;
+    const float tz = a.pz - bp.z;
+    const float dist = sqrtf(tx * tx + ty * ty + tz * tz);
+    if (dist <= edge) {
+        const float nx = tx / dist;
+        const float ny = ty / dist;
+        const float nz = tz / dist;
+        if (!SPRING) {
+            const uint4 q = vfme_i[bo];
+            const float bvx = half_lo(q.x), bvy = half_hi(q.x), bvz = half_lo(q.y);
+            const float m1 = a.m;
+            const float m2 = half_lo(q.w);
+            const float rx = a.vx - bvx;
+            const float ry = a.vy - bvy;
+            const float rz = a.vz - bvz;
+            const float rspd = rx * nx + ry * ny + rz * nz;
+            if (rspd < 0.0f) {
+                const float rnx = rspd * nx, rny = rspd * ny, rnz = rspd * nz;
+                const float coef = m1 - c.e_agent * m2;
+                const float msum = m1 + m2;
+                const float rcx = (coef * rnx) / msum;
+                const float rcy = (coef * rny) / msum;
+                const float rcz = (coef * rnz) / msum;
+                a.vx = (bvx + (rx - rnx)) + rcx;
+                a.vy = (bvy + (ry - rny)) + rcy;
+                a.vz = (bvz + (rz - rnz)) + rcz;
+            }
+            const float push = 0.5f * (edge - dist);
+            a.px = a.px + push * nx;
+            a.py = a.py + push * ny;
+            a.pz = a.pz + push * nz;
+        } else {
+            const float makeup = edge - dist;
+            const float kf = c.k_agent * makeup;
+            a.fx = a.fx + (kf * nx + c.neg_c_over_dt * a.vx);
+            a.fy = a.fy + (kf * ny + c.neg_c_over_dt * a.vy);
+            a.fz = a.fz + (kf * nz + c.neg_c_over_dt * a.vz);
+            const float limit = makeup - c.r_agent * a.scale;
+            if (limit > 0.0f) {
+                a.px = a.px + limit * nx;
+                a.py = a.py + limit * ny;
+                a.pz = a.pz + limit * nz;
+            }
+        }
+    }
+}
+
+template <bool SPRING>
+__device__ __forceinline__ void collide_range(const PhysConsts &c, Agent &a, uint32_t first,
+                                              uint32_t last, uint32_t skip,
+                                              const uint32_t *__restrict__ perm,
+                                              const float4 *__restrict__ pos_i,
+                                              const uint4 *__restrict__ vfme_i)
+{
+    // `for (var i = first; i <= last; i++)`; first == INVALID means empty.
+    if (first == kInvalid) return;
+    for (uint32_t i = first; i <= last; ++i) {
+        if (i != skip) {
+            const uint32_t bo = perm[i];
+            solve_sphere<SPRING>(c, a, pos_i[bo], vfme_i, bo);
+        }
+        if (i == 0xFFFFFFFEu) break;
+    }
+}
+
+// handle_collision (physics.wgsl:171-206): one neighbouring row.
+template <bool SPRING>
+__device__ __forceinline__ void handle_row(const PhysConsts &c, Agent &a, uint32_t cx,
+                                           uint32_t base, const uint32_t *__restrict__ start,
+                                           const uint32_t *__restrict__ close,
+                                           const uint32_t *__restrict__ perm,
+                                           const float4 *__restrict__ pos_i,
+                                           const uint4 *__restrict__ vfme_i)
+{
+    uint32_t first = kInvalid, last = 0u;
+    if (cx > 0u) {
+        const uint32_t st = start[base - 1u];
+        if (st != kInvalid) {
+            first = st;
+            last = close[base - 1u];
+        }
+    }
+    {
+        const uint32_t st = start[base];
+        if (st != kInvalid) {
+            if (first == kInvalid) first = st;
+            last = close[base];
+        }
+    }
+    if (cx < c.grids - 1u) {
+        const uint32_t st = start[base + 1u];
+        if (st != kInvalid) {
+            if (first == kInvalid) first = st;
+            last = close[base + 1u];
+        }
+    }
+    collide_range<SPRING>(c, a, first, last, kInvalid, perm, pos_i, vfme_i);
+}
+
+// solve_cube_constraint (physics.wgsl:269-298).
+template <bool SPRING>
+__device__ __forceinline__ void solve_cube(const PhysConsts &c, Agent &a)
+{
+    const float ex = c.half_map_size - a.scale;
+    const float ey = c.half_map_height - a.scale;
+    const float ez = c.half_map_size - a.scale;
+    const float tx = a.px - 0.0f;
+    const float ty = a.py - c.half_map_height;
+    const float tz = a.pz - 0.0f;
+    const float mx = ex - fabsf(tx);
+    const float my = ey - fabsf(ty);
+    const float mz = ez - fabsf(tz);
+    const bool ix = mx < 0.0f, iy = my < 0.0f, iz = mz < 0.0f;
+    if (!(ix || iy || iz)) return;
+    if (!SPRING) {
+        a.px = clamp_sel(tx, -ex, ex) + 0.0f;
+        a.py = clamp_sel(ty, -ey, ey) + c.half_map_height;
+        a.pz = clamp_sel(tz, -ez, ez) + 0.0f;
+        a.vx = a.vx * (ix ? -c.e_envir : 1.0f);
+        a.vy = a.vy * (iy ? -c.e_envir : 1.0f);
+        a.vz = a.vz * (iz ? -c.e_envir : 1.0f);
+    } else {
+        const float mm = -(a.scale * c.r_agent);
+        a.fx = a.fx + (c.k_agent * sign_sel(tx) * clamp_sel(mx, mm, 0.0f) + c.neg_c_over_dt * a.vx);
+        a.fy = a.fy + (c.k_agent * sign_sel(ty) * clamp_sel(my, mm, 0.0f) + c.neg_c_over_dt * a.vy);
+        a.fz = a.fz + (c.k_agent * sign_sel(tz) * clamp_sel(mz, mm, 0.0f) + c.neg_c_over_dt * a.vz);
+        const bool ux = mx < mm, uy = my < mm, uz = mz < mm;
+        if (ux || uy || uz) {
+            const float e2x = ex - mm, e2y = ey - mm, e2z = ez - mm;
+            a.px = clamp_sel(tx, -e2x, e2x) + 0.0f;
+            a.py = clamp_sel(ty, -e2y, e2y) + c.half_map_height;
+            a.pz = clamp_sel(tz, -e2z, e2z) + 0.0f;
+            const float dvx = c.neg_1pe_over_dt * (ux ? a.vx : 0.0f);
+            const float dvy = c.neg_1pe_over_dt * (uy ? a.vy : 0.0f);
+            const float dvz = c.neg_1pe_over_dt * (uz ? a.vz : 0.0f);
+            a.fx = a.fx + a.m * (dvx / c.t_delta);
+            a.fy = a.fy + a.m * (dvy / c.t_delta);
+            a.fz = a.fz + a.m * (dvz / c.t_delta);
+        }
+    }
+}
+
+// main() with LOOP_PIPELINE (physics.wgsl:57-90) for sorted slot s.
+template <bool SPRING>
+__global__ __launch_bounds__(256) void physics_loop_kernel(
+    PhysConsts c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ start,
+    const uint32_t *__restrict__ close, uint32_t *__restrict__ key_out,
+    const float4 *__restrict__ pos_i, const uint4 *__restrict__ vfme_i,
+    float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o, uint32_t n)
+{
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t o = perm[s];
+    Agent a = agent_from(pos_i[o], vfme_i[o]);
+    uint32_t cx, cz;
+    cell_of(c, a.px, a.pz, cx, cz);
+    const uint32_t k = cz * c.grids + cx;
+
+    // verlet_integration (physics.wgsl:262-266)
+    {
+        const float ax = a.fx / a.m, ay = a.fy / a.m, az = a.fz / a.m;
+        a.vx = a.vx + ((c.gravity_x + ax) * c.t_delta) * c.t_delta;
+        a.vy = a.vy + ((c.gravity_y + ay) * c.t_delta) * c.t_delta;
+        a.vz = a.vz + ((c.gravity_z + az) * c.t_delta) * c.t_delta;
+        a.px = a.px + a.vx;
+        a.py = a.py + a.vy;
+        a.pz = a.pz + a.vz;
+        a.fx = 0.0f;
+        a.fy = 0.0f;
+        a.fz = 0.0f;
+    }
+
+    // handle_collisions (physics.wgsl:119-169)
+    const float ox = a.px, oy = a.py, oz = a.pz;
+    {
+        uint32_t first = start[k];
+        uint32_t last = close[k];
+        if (cx > 0u) {
+            const uint32_t st = start[k - 1u];
+            if (st != kInvalid) first = st;
+        }
+        if (cx < c.grids - 1u) {
+            const uint32_t st = start[k + 1u];
+            if (st != kInvalid) last = close[k + 1u];
+        }
+        collide_range<SPRING>(c, a, first, last, s, perm, pos_i, vfme_i);
+    }
+    if (cz > 0u) handle_row<SPRING>(c, a, cx, k - c.grids, start, close, perm, pos_i, vfme_i);
+    if (cz < c.grids - 1u)
+        handle_row<SPRING>(c, a, cx, k + c.grids, start, close, perm, pos_i, vfme_i);
+    {
+        const float dx = a.px - ox, dy = a.py - oy, dz = a.pz - oz;
+        if (sqrtf(dx * dx + dy * dy + dz * dz) > a.scale) {
+            // hash3f1u (physics.wgsl:334-337)
+            const uint32_t v = s * (s ^ (s >> 15));
+            const float jx = ((float)(v * 0x1u) / 4294967296.0f) * 2.0f - 1.0f;
+            const float jy = ((float)(v * 0x1FFu) / 4294967296.0f) * 2.0f - 1.0f;
+            const float jz = ((float)(v * 0x3FFFFu) / 4294967296.0f) * 2.0f - 1.0f;
+            const float len = sqrtf(jx * jx + jy * jy + jz * jz);
+            a.px = a.px + jx / len;
+            a.py = a.py + jy / len;
+            a.pz = a.pz + jz / len;
+        }
+    }
+
+    solve_cube<SPRING>(c, a);
+
+    a.vx = clamp_sel(a.vx, -65504.0f, 65504.0f);
+    a.vy = clamp_sel(a.vy, -65504.0f, 65504.0f);
+    a.vz = clamp_sel(a.vz, -65504.0f, 65504.0f);
+    a.fx = clamp_sel(a.fx, -65504.0f, 65504.0f);
+    a.fy = clamp_sel(a.fy, -65504.0f, 65504.0f);
+    a.fz = clamp_sel(a.fz, -65504.0f, 65504.0f);
+
+    pos_o[s] = make_float4(a.px, a.py, a.pz, a.scale);
+    vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
+                           pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
+    uint32_t nx, nz;
+    cell_of(c, a.px, a.pz, nx, nz);
+    key_out[s] = nz * c.grids + nx;
+}
+
+// main() with INIT_PIPELINE (physics.wgsl:91-96).
+__global__ __launch_bounds__(256) void physics_init_kernel(
+    PhysConsts c, const uint32_t *__restrict__ perm, uint32_t *__restrict__ key_out,
+    const float4 *__restrict__ pos_i, const uint4 *__restrict__ vfme_i,
+    float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o, uint32_t n)
+{
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t o = perm[s];
+    const float4 p = pos_i[o];
+    const Agent a = agent_from(p, vfme_i[o]);
+    uint32_t cx, cz;
+    cell_of(c, a.px, a.pz, cx, cz);
+    pos_o[s] = p;
+    vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
+                           pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
+    key_out[s] = cz * c.grids + cx;
+}
+
+__global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *__restrict__ pos,
+                                                    uint32_t *__restrict__ key_out, uint32_t n)
+{
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= n) return;
+    const float4 p = pos[s];
+    uint32_t cx, cz;
+    cell_of(c, p.x, p.z, cx, cz);
+    key_out[s] = cz * c.grids + cx;
+}
+
+hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
+                          const uint32_t *start, const uint32_t *close, uint32_t *key_out,
+                          const float4 *pos_i, const uint4 *vfme_i, float4 *pos_o,
+                          uint4 *vfme_o, uint32_t n, hipStream_t stream)
+{
+    if (n == 0) return hipSuccess;
+    const dim3 grid((n + 255u) / 256u);
+    if (init) {
+        hipLaunchKernelGGL(physics_init_kernel, grid, dim3(256), 0, stream, c, perm, key_out,
+                           pos_i, vfme_i, pos_o, vfme_o, n);
+    } else if (c.enable_spring) {
+        hipLaunchKernelGGL(physics_loop_kernel<true>, grid, dim3(256), 0, stream, c, perm, start,
+                           close, key_out, pos_i, vfme_i, pos_o, vfme_o, n);
+    } else {
+        hipLaunchKernelGGL(physics_loop_kernel<false>, grid, dim3(256), 0, stream, c, perm,
+                           start, close, key_out, pos_i, vfme_i, pos_o, vfme_o, n);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_rekey(const PhysConsts &c, const float4 *pos, uint32_t *key_out, uint32_t n,
+                        hipStream_t stream)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(rekey_kernel, dim3((n + 255u) / 256u), dim3(256), 0, stream, c, pos,
+                       key_out, n);
+    return hipGetLastError();
+}
+
+}  // namespace mb

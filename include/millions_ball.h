@@ -1,0 +1,176 @@
+/*
+ * millions_ball.h -- C ABI of the MI355X (gfx950) physics step of
+ * bevy_millions_ball.
+ *
+ * The library (bevy_millions_ball_amd/libmillions_ball.so) replaces the
+ * reference's compute-shader path: bevy_radix_sort -> spatial_hashing.wgsl ->
+ * physics.wgsl.  Every entry point takes plain pointers and sizes; `stream` is
+ * a hipStream_t passed as void* (NULL = the default stream).  All work is
+ * stream ordered; functions that copy to or from host memory synchronise the
+ * stream before returning.  Handles are not thread safe.
+ *
+ * Return codes: MB_OK (0) or a negative MB_ERR_*; mb_last_error() gives the
+ * message of the calling thread's last failure.  The reference panics or
+ * unwraps in the same places (src/physics/mod.rs:788-798, 633-657).
+ *
+ * Citations are to the reference tree (src/...).
+ */
+#ifndef MILLIONS_BALL_H
+#define MILLIONS_BALL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MB_OK 0
+#define MB_ERR_INVALID_ARGUMENT (-1)
+#define MB_ERR_HIP (-2)
+#define MB_ERR_OUT_OF_MEMORY (-3)
+#define MB_ERR_STATE (-4)
+
+/* 0xFFFFFFFF marks an empty cell in `start` (spatial_hashing.wgsl:39-42). */
+#define MB_INVALID_INDEX 0xFFFFFFFFu
+
+/* Mirrors PhysicsPlugin / PhysicsSettings (src/physics/mod.rs:53-85,
+ * 294-362).  Booleans are int32 (0 / 1). */
+typedef struct mb_settings {
+    int32_t enable_simulation;
+    int32_t enable_spring;
+    uint32_t number_of_grids_one_dimension;
+    uint32_t grid_size;
+    uint32_t half_map_height;
+    float gravity[3];
+    float e_envir;
+    float e_agent;
+    float k_agent;
+    float c_agent;
+    float r_agent;
+    uint32_t max_number_of_agents;
+    float time_delta;
+} mb_settings;
+
+/* Device addresses of the ping-pong buffers (EVE = index 0, ODD = index 1).
+ * Replaces the ShaderStorageBuffer handles of src/physics/mod.rs:37-51 and the
+ * bevy_radix_sort EVE/ODD global keys/vals handles (mod.rs:27-32).  Each
+ * position_with_scale buffer holds N float4 (xyz, w = radius); each
+ * packed_vfme buffer holds N uint4 of pack2x16float pairs
+ * (vx|vy, vz|fx, fy|fz, mass|0), physics.wgsl:235-242. */
+typedef struct mb_buffers {
+    void *position_with_scale[2];
+    void *packed_vfme[2];
+    void *global_keys[2];
+    void *global_vals[2];
+    uint32_t number_of_agents;
+} mb_buffers;
+
+typedef struct mb_physics mb_physics;
+
+/* ---- Plugin surface ---------------------------------------------------- */
+
+/* PhysicsPlugin::default (mod.rs:87-109). */
+void mb_settings_default(mb_settings *settings);
+
+/* PhysicsPlugin::build + create_shader_storage_buffers (mod.rs:111-125,
+ * 241-292): allocates every device buffer for max_number_of_agents spheres on
+ * the current HIP device.  Rejects the inputs the reference documents as
+ * unsupported: number_of_grids_one_dimension^2 > max_number_of_agents
+ * (mod.rs:351-362), number_of_grids_one_dimension^2 > 2^24 (mod.rs:305-320),
+ * a zero grid count or grid size, max_number_of_agents outside [1, 2^30). */
+int mb_physics_create(const mb_settings *settings, mb_physics **out);
+int mb_physics_destroy(mb_physics *physics);
+
+/* Live settings change (ResMut<PhysicsSettings> -> PhysicsNode::update,
+ * mod.rs:808-837).  max_number_of_agents cannot change (the buffers are sized
+ * once, as in the reference).  `time` keeps counting; a change of the grid
+ * (count, cell size) re-keys the current state before the next frame. */
+int mb_physics_set_settings(mb_physics *physics, const mb_settings *settings);
+int mb_physics_get_settings(const mb_physics *physics, mb_settings *settings);
+
+/* Writes N spheres of host state into the EVE state buffers and zeroes the
+ * sort keys (the buffers' state at PhysicsInnerPlugin::build), then resets
+ * time to -1 so that the next frame runs the INIT pipeline.  Rejects
+ * non-finite positions, radii or velocities. */
+int mb_physics_upload_state(mb_physics *physics, const float *position_with_scale,
+                            const uint32_t *packed_vfme, void *stream);
+/* Same from device memory (N float4 + N uint4, no validation). */
+int mb_physics_upload_state_device(mb_physics *physics,
+                                   const void *position_with_scale,
+                                   const void *packed_vfme, void *stream);
+/* Copies the current state (the output of the last frame, in sorted order)
+ * to host memory. */
+int mb_physics_download_state(const mb_physics *physics, float *position_with_scale,
+                              uint32_t *packed_vfme, void *stream);
+
+/* Runs `frames` frames of PhysicsNode::run (mod.rs:846-912): radix sort ->
+ * spatial hashing -> physics (INIT when time == 0, else LOOP).  Does nothing
+ * while enable_simulation is 0 (mod.rs:780-782, 853-855). */
+int mb_physics_step(mb_physics *physics, uint32_t frames, void *stream);
+
+/* PhysicsBindGroup::time / is_even (mod.rs:717-729): time of the last frame
+ * run (-1 before the first); is_even returns 1, 0, or -1 when not started. */
+int64_t mb_physics_time(const mb_physics *physics);
+int mb_physics_is_even(const mb_physics *physics);
+
+/* Device addresses of the ping-pong buffers, for a renderer that reads
+ * position_with_scale by parity (src/custom_materials/physics_ball/mod.rs:
+ * 709-756).  The state of the last frame lives at index (time + 1) & 1. */
+int mb_physics_buffers(const mb_physics *physics, mb_buffers *out);
+
+/* Debug views of the last frame: perm (sorted slot -> index into the input
+ * state), start/close cell ranges (max_number_of_agents entries each; close is
+ * meaningful only where start != MB_INVALID_INDEX) and the cell keys written
+ * for the next frame.  Any pointer may be NULL. */
+int mb_physics_debug_download(const mb_physics *physics, uint32_t *perm,
+                              uint32_t *start, uint32_t *close,
+                              uint32_t *next_keys, void *stream);
+
+/* Per-frame device time of each kernel family (sort, spatial hashing,
+ * physics), measured with HIP events recorded on the frame's stream while
+ * profiling is enabled.  Recording does not block.  mb_physics_profile_read
+ * waits for the recorded frames, writes up to `capacity` rows of three
+ * milliseconds values ([frame][sort, hashing, physics]) to stage_ms (may be
+ * NULL to only count), stores the row count in *frames and forgets them. */
+int mb_physics_set_profiling(mb_physics *physics, int enable);
+int mb_physics_profile_read(mb_physics *physics, double *stage_ms, uint32_t capacity,
+                            uint32_t *frames);
+
+const char *mb_last_error(void);
+
+/* ---- Stage entry points (device pointers) ----------------------------- */
+
+/* bevy_radix_sort::run(encoder, .., number_of_keys, pass_range, init_vals,
+ * read_from_even) (mod.rs:880-890; test call spatial_hashing.rs:456-479):
+ * stable LSD sort by 8-bit digits over passes [pass_begin, pass_end).
+ * vals_in == NULL generates the identity values (init_vals = true).  Output
+ * goes to keys_out / vals_out (the reference's EVE/ODD parity is the caller's
+ * choice of these pointers).  `workspace` holds
+ * mb_radix_sort_workspace_size(n) bytes of device memory, zero-filled before
+ * its first use; calls that share a workspace must not overlap. */
+size_t mb_radix_sort_workspace_size(uint32_t max_number_of_keys);
+int mb_radix_sort(void *workspace, const uint32_t *keys_in, const uint32_t *vals_in,
+                  uint32_t *keys_out, uint32_t *vals_out, uint32_t number_of_keys,
+                  uint32_t pass_begin, uint32_t pass_end, void *stream);
+
+/* spatial_hashing::run(.., number_of_keys, read_from_even)
+ * (spatial_hashing.rs:191-243): start[0..zero_count) = MB_INVALID_INDEX, then
+ * start[k] / close[k] for the first / last sorted index of every key k.  Keys
+ * >= zero_count are skipped (the reference writes out of bounds there). */
+int mb_spatial_hashing(const uint32_t *sorted_keys, uint32_t *start, uint32_t *close,
+                       uint32_t number_of_keys, uint32_t zero_count, void *stream);
+
+/* physics::run(.., number_of_agents) (mod.rs:915-956) with the bind group of
+ * one parity (mod.rs:659-705) spelled out: init != 0 selects INIT_PIPELINE.
+ * start / close must cover number_of_grids_one_dimension^2 cells. */
+int mb_physics_run(const mb_settings *settings, int init, const uint32_t *perm,
+                   const uint32_t *start, const uint32_t *close, uint32_t *key_out,
+                   const void *position_with_scale_in, const void *packed_vfme_in,
+                   void *position_with_scale_out, void *packed_vfme_out,
+                   uint32_t number_of_agents, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

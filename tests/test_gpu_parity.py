@@ -1,0 +1,304 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle.
+
+Integer stages (cell keys, sorted permutation, start/close) must be bit-exact.
+State is compared bit-exactly too: the kernels follow the oracle's operation
+order, so any difference is a bug (the stated bound is 1e-5 relative for f32
+positions and unpacked f16 velocity/force; `assert_state_close` reports it
+when bit-equality fails so the size of a mismatch is visible)."""
+import json
+from dataclasses import replace
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from bevy_millions_ball_amd import Physics, PhysicsSettings, radix_sort_run, \
+    radix_sort_workspace_size, spatial_hashing_run, physics_run
+from bevy_millions_ball_amd.scenes import CONFIGS, SceneConfig, generate, pack_vfme, unpack_vfme
+
+pytestmark = pytest.mark.gpu
+
+KAT = json.loads((Path(__file__).parent / "golden" / "spatial_hashing_kat.json").read_text())
+INVALID = 0xFFFFFFFF
+
+
+def dev(a, device):
+    import torch
+
+    a = np.ascontiguousarray(a)
+    return torch.from_numpy(a.view(np.int32) if a.dtype == np.uint32 else a).to(device)
+
+
+def host_u32(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+def assert_state_close(pos_a, vf_a, pos_b, vf_b, rtol=1e-5, atol=1e-6):
+    if np.array_equal(pos_a.view(np.uint32), pos_b.view(np.uint32)) and np.array_equal(vf_a, vf_b):
+        return
+    va, fa, ma = unpack_vfme(vf_a)
+    vb, fb, mb = unpack_vfme(vf_b)
+    for name, x, y in (("pos", pos_a, pos_b), ("vel", va, vb), ("force", fa, fb), ("mass", ma, mb)):
+        bad = ~np.isclose(x, y, rtol=rtol, atol=atol, equal_nan=True)
+        assert not bad.any(), f"{name}: {bad.sum()} mismatches, first {np.argwhere(bad)[:3].tolist()}"
+    raise AssertionError("state differs at bit level (within 1e-5 relative)")
+
+
+# ---------------------------------------------------------------------------
+# Stage entry points
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("case", KAT["cases"],
+                         ids=lambda c: f"n{c['number_of_keys']}-p{c['pass_count']}-{c['read_from_even']}")
+def test_sort_and_hashing_reference_known_answers(gpu, case):
+    import torch
+
+    n = case["number_of_keys"]
+    print(f"[kat] n={n} passes={case['pass_count']}", flush=True)
+    keys = dev(np.arange(n, dtype=np.uint32) % 256, gpu)
+    vals = dev(np.arange(n, dtype=np.uint32), gpu)
+    ko = torch.empty_like(keys)
+    vo = torch.empty_like(keys)
+    ws = torch.zeros(radix_sort_workspace_size(n), dtype=torch.uint8, device=gpu)
+    radix_sort_run(ws, keys, vals, ko, vo, n, range(0, case["pass_count"]))
+    start = torch.empty(max(n, 1), dtype=torch.int32, device=gpu)
+    close = torch.empty_like(start)
+    spatial_hashing_run(ko, start, close, n)
+    cells = len(case["start"])
+    assert host_u32(start)[:cells].tolist() == case["start"]
+    assert host_u32(close)[:cells].tolist() == case["close"]
+    perm = host_u32(vo)
+    assert np.array_equal(perm, np.argsort(np.arange(n) % 256, kind="stable").astype(np.uint32))
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 255, 4095, 4096, 4097, 12_345, 100_000, 1_048_576])
+@pytest.mark.parametrize("passes", [1, 3, 4])
+def test_sort_random_matches_stable_argsort(gpu, n, passes):
+    import torch
+
+    rng = np.random.default_rng(n * 7 + passes)
+    keys = rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+    vals = rng.permutation(n).astype(np.uint32)
+    ws = torch.zeros(radix_sort_workspace_size(n), dtype=torch.uint8, device=gpu)
+    kd = dev(keys, gpu)
+    vd = dev(vals, gpu)
+    ko = torch.empty_like(kd)
+    vo = torch.empty_like(kd)
+    mask = np.uint32(0xFFFFFFFF if passes == 4 else (1 << (8 * passes)) - 1)
+    order = np.argsort(keys & mask, kind="stable")
+    for rep in range(2):  # the workspace is reused: it must clean up after itself
+        radix_sort_run(ws, kd, vd, ko, vo, n, range(0, passes))
+        assert np.array_equal(host_u32(ko), keys[order])
+        assert np.array_equal(host_u32(vo), vals[order])
+    radix_sort_run(ws, kd, None, ko, vo, n, range(0, passes))  # init_vals = true
+    assert np.array_equal(host_u32(vo), order.astype(np.uint32))
+
+
+def test_sort_nearly_sorted_skewed_keys(gpu):
+    """Keys like a physics frame: nearly sorted, long runs of equal high bytes."""
+    import torch
+
+    rng = np.random.default_rng(9)
+    n = 300_000
+    keys = np.sort(rng.integers(0, 1 << 20, n)).astype(np.uint32)
+    swap = rng.integers(0, n - 1, n // 10)
+    keys[swap], keys[swap + 1] = keys[swap + 1], keys[swap]
+    keys[rng.integers(0, n, 1000)] = 0
+    ws = torch.zeros(radix_sort_workspace_size(n), dtype=torch.uint8, device=gpu)
+    kd = dev(keys, gpu)
+    ko = torch.empty_like(kd)
+    vo = torch.empty_like(kd)
+    radix_sort_run(ws, kd, None, ko, vo, n, range(0, 3))
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(host_u32(vo), order.astype(np.uint32))
+
+
+def test_spatial_hashing_random(gpu, oracle):
+    import torch
+
+    rng = np.random.default_rng(4)
+    n = 200_000
+    keys = np.sort(rng.integers(0, 150_000, n)).astype(np.uint32)
+    start = torch.zeros(n, dtype=torch.int32, device=gpu)
+    close = torch.zeros(n, dtype=torch.int32, device=gpu)
+    spatial_hashing_run(dev(keys, gpu), start, close, n)
+    s_ref, c_ref = oracle.spatial_hashing(keys, start=np.zeros(n, np.uint32), close=np.zeros(n, np.uint32))
+    s = host_u32(start)
+    c = host_u32(close)
+    assert np.array_equal(s, s_ref)
+    valid = s_ref != INVALID
+    assert np.array_equal(c[valid], c_ref[valid])
+
+
+@pytest.mark.parametrize("spring", [False, True])
+def test_physics_run_stage_matches_oracle(gpu, oracle, spring):
+    cfg = replace(CONFIGS["plain_65k"], number_of_agents=8192, grids=64, enable_spring=spring,
+                  k_agent=500.0, c_agent=5.0)
+    ps = cfg.settings()
+    pos, vf = generate(cfg, seed=21)
+    n = cfg.number_of_agents
+    # a sort + hash of the INIT keys gives the LOOP inputs
+    c = oracle.compile_consts(ps)
+    keys = np.array([oracle.cell_key(c, x, z) for x, z in zip(pos[:, 0], pos[:, 2])], np.uint32)
+    sk, perm = oracle.radix_sort(keys, None, range(0, 3))
+    start, close = oracle.spatial_hashing(sk, start=np.zeros(n, np.uint32), close=np.zeros(n, np.uint32))
+    for init in (True, False):
+        k_ref, p_ref, v_ref = oracle.physics_run(ps, init, perm, start, close, pos, vf)
+        import torch
+
+        ko = torch.empty(n, dtype=torch.int32, device=gpu)
+        po = torch.empty((n, 4), dtype=torch.float32, device=gpu)
+        vo = torch.empty((n, 4), dtype=torch.int32, device=gpu)
+        physics_run(ps, init, dev(perm, gpu), dev(start, gpu), dev(close, gpu), ko,
+                    dev(pos, gpu), dev(vf, gpu), po, vo, n)
+        assert np.array_equal(host_u32(ko), k_ref)
+        assert_state_close(po.cpu().numpy(), host_u32(vo).reshape(n, 4), p_ref, v_ref)
+
+
+# ---------------------------------------------------------------------------
+# Whole frames through mb_physics_step
+# ---------------------------------------------------------------------------
+
+def compare_frames(gpu_world, ref_world, frames, check_every=1):
+    for f in range(frames):
+        gpu_world.run(1)
+        ref_world.step(1)
+        assert gpu_world.time() == ref_world.time()
+        if (f + 1) % check_every and f != frames - 1:
+            continue
+        d = gpu_world.debug()
+        r = ref_world.debug()
+        assert np.array_equal(d["perm"], r["perm"]), f"perm differs at frame {gpu_world.time()}"
+        assert np.array_equal(d["start"], r["start"]), f"start differs at frame {gpu_world.time()}"
+        valid = r["start"] != INVALID
+        cells = np.nonzero(valid)[0]
+        assert np.array_equal(d["close"][cells], r["close"][cells])
+        assert np.array_equal(d["next_keys"], r["next_keys"])
+        pg, vg = gpu_world.download_state()
+        pr, vr = ref_world.download()
+        assert_state_close(pg, vg, pr, vr)
+
+
+def make_pair(oracle, cfg: SceneConfig, seed):
+    ps = cfg.settings()
+    pos, vf = generate(cfg, seed=seed)
+    g = Physics(ps)
+    g.upload_state(pos, vf)
+    r = oracle.World(ps)
+    r.upload(pos, vf)
+    return g, r
+
+
+@pytest.mark.parametrize("name,n,grids,frames", [
+    ("plain_65k", 4096, 64, 40),
+    ("plain_65k", 65_536, 256, 12),
+    ("spring_clumps_1m", 16_384, 128, 40),
+    ("uniform_1m", 20_000, 128, 20),   # N not a multiple of the tile
+])
+def test_step_bit_exact_small(gpu, oracle, name, n, grids, frames):
+    cfg = replace(CONFIGS[name], number_of_agents=n, grids=grids)
+    print(f"[step] {name} n={n} G={grids} frames={frames}", flush=True)
+    g, r = make_pair(oracle, cfg, seed=n)
+    with g:
+        compare_frames(g, r, frames)
+
+
+@pytest.mark.parametrize("name,frames", [("uniform_1m", 3), ("spring_clumps_1m", 3),
+                                         ("uniform_4m", 2)])
+def test_step_bit_exact_full_size(gpu, oracle, name, frames):
+    cfg = CONFIGS[name]
+    print(f"[full] {name} frames={frames}", flush=True)
+    g, r = make_pair(oracle, cfg, seed=1)
+    with g:
+        compare_frames(g, r, frames)
+
+
+def test_hundred_frames_statistics(gpu, oracle):
+    """Over 100 frames the trajectories stay bit-identical; the kinetic energy
+    and penetration statistics are then trivially equal, and are checked for
+    being finite and bounded as a sanity property."""
+    cfg = replace(CONFIGS["plain_65k"], number_of_agents=16_384, grids=128)
+    g, r = make_pair(oracle, cfg, seed=100)
+    with g:
+        g.run(101)
+        r.step(101)
+        pg, vg = g.download_state()
+        pr, vr = r.download()
+        assert_state_close(pg, vg, pr, vr)
+        v, _, m = unpack_vfme(vg)
+        ke = 0.5 * float(np.sum(m[:, None] * v.astype(np.float64) ** 2))
+        assert np.isfinite(ke) and ke < 16_384 * 3
+
+
+def test_time_parity_and_buffers(gpu, oracle):
+    cfg = replace(CONFIGS["plain_65k"], number_of_agents=4096, grids=64)
+    g, r = make_pair(oracle, cfg, seed=2)
+    with g:
+        assert g.time() == -1 and g.is_even() is None
+        g.run(1)
+        assert g.time() == 0 and g.is_even() is True
+        g.run(1)
+        assert g.time() == 1 and g.is_even() is False
+        b = g.buffers()
+        assert b.number_of_agents == 4096
+        assert len({b.position_with_scale[0], b.position_with_scale[1], b.packed_vfme[0],
+                    b.packed_vfme[1]}) == 4
+
+
+def test_settings_change_and_disable(gpu, oracle):
+    cfg = replace(CONFIGS["plain_65k"], number_of_agents=4096, grids=64)
+    g, r = make_pair(oracle, cfg, seed=3)
+    with g:
+        compare_frames(g, r, 3)
+        ps = cfg.settings()
+        ps.enable_spring = True
+        ps.k_agent, ps.c_agent = 500.0, 5.0
+        ps.e_envir = 0.5
+        g.set_settings(ps)
+        r.set_settings(ps)
+        compare_frames(g, r, 3)
+        ps.enable_simulation = False
+        g.set_settings(ps)
+        t = g.time()
+        g.run(5)
+        assert g.time() == t
+
+
+def test_edge_cases_tiny_worlds(gpu, oracle):
+    rng = np.random.default_rng(8)
+    # one sphere, one cell
+    for n, grids in ((1, 1), (64, 1), (256, 16), (257, 16)):
+        ps = PhysicsSettings(number_of_grids_one_dimension=grids, grid_size=2, half_map_height=4,
+                             e_agent=0.8, e_envir=0.8, time_delta=1 / 60,
+                             _max_number_of_agents=n)
+        hms = grids  # G * 2 / 2
+        pos = np.empty((n, 4), np.float32)
+        pos[:, 0] = rng.uniform(-hms + 0.5, hms - 0.5, n)
+        pos[:, 2] = rng.uniform(-hms + 0.5, hms - 0.5, n)
+        pos[:, 1] = rng.uniform(0.5, 7.5, n)
+        pos[:, 3] = 0.5
+        vf = pack_vfme(rng.uniform(-1, 1, (n, 3)).astype(np.float32), np.zeros((n, 3), np.float32),
+                       np.ones(n, np.float32))
+        g = Physics(ps)
+        g.upload_state(pos, vf)
+        r = oracle.World(ps)
+        r.upload(pos, vf)
+        with g:
+            compare_frames(g, r, 8)
+
+
+def test_out_of_box_upload_is_clamped_like_oracle(gpu, oracle):
+    """The reference demo's own start lattice lies partly outside its box
+    (main.rs:43-44 with mod.rs:173-196); cell ids are clamped to the grid."""
+    from bevy_millions_ball_amd.scenes import reference_lattice
+
+    n = 64 * 64
+    ps = PhysicsSettings(number_of_grids_one_dimension=64, grid_size=1, half_map_height=8,
+                         e_agent=0.8, e_envir=0.8, _max_number_of_agents=n)
+    pos, vf = reference_lattice(n)
+    g = Physics(ps)
+    g.upload_state(pos, vf)
+    r = oracle.World(ps)
+    r.upload(pos, vf)
+    with g:
+        compare_frames(g, r, 4)
