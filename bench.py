@@ -33,14 +33,18 @@ HBM_PEAK_GBS = 8000.0
 
 def physics_bytes_per_sphere(n: int, cells: int) -> float:
     """Algorithmic HBM bytes of one collide + integrate launch per sphere:
-    perm 4 + state in 32 + cell ranges 8 * C / N + state out 32 + key out 4."""
-    return 72.0 + 8.0 * cells / n
+    perm 4 + state in 32 + cell offsets 4 * C / N + state out 32 + key out 4."""
+    return 72.0 + 4.0 * cells / n
 
 
-def sort_bytes_per_sphere() -> float:
-    """Compulsory bytes of the 3-pass 8-bit stable sort with generated values:
-    4 (digit counts) + 12 + 16 + 16 per key."""
-    return 48.0
+def partition_bytes_per_sphere() -> float:
+    """Row partition: key in 4 + (column, index) pair out 8."""
+    return 12.0
+
+
+def row_sort_bytes_per_sphere(n: int, cells: int) -> float:
+    """Row sort: pair in 8 + perm out 4 + cell offsets out 4 * C / N."""
+    return 12.0 + 4.0 * cells / n
 
 
 def cpu_baseline(cfg, pos, vf, budget_s: float, threads: int) -> dict:
@@ -128,7 +132,8 @@ def main() -> int:
 
     phys_bytes = physics_bytes_per_sphere(n, cells) * n
     achieved = phys_bytes / (stage_ms[2] * 1e-3) / 1e9
-    sort_gbs = sort_bytes_per_sphere() * n / (stage_ms[0] * 1e-3) / 1e9
+    part_gbs = partition_bytes_per_sphere() * n / (stage_ms[0] * 1e-3) / 1e9
+    rsort_gbs = row_sort_bytes_per_sphere(n, cells) * n / (stage_ms[1] * 1e-3) / 1e9
 
     traffic = None
     tfile = ROOT / "profiles" / f"traffic_{cfg.name}.json"
@@ -167,9 +172,10 @@ def main() -> int:
         "p99_step_ms": float(np.percentile(lat, 99)),
         "device_step_ms_p50": float(np.percentile(device_step_ms, 50)),
         "device_step_ms_p99": float(np.percentile(device_step_ms, 99)),
-        "stage_ms": {"sort": float(stage_ms[0]), "spatial_hashing": float(stage_ms[1]),
+        "stage_ms": {"row_partition": float(stage_ms[0]), "row_sort": float(stage_ms[1]),
                      "collide_integrate": float(stage_ms[2])},
-        "sort_achieved_GBs": sort_gbs,
+        "stage_achieved_GBs": {"row_partition": part_gbs, "row_sort": rsort_gbs,
+                               "collide_integrate": achieved},
         "roofline": {
             "kernel": "physics_loop_kernel (collide + integrate)",
             "bound": "hbm",

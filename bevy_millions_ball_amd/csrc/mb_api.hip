@@ -118,15 +118,6 @@ int validate_settings(const mb_settings &s)
     return MB_OK;
 }
 
-// 8-bit passes needed to sort keys below `cells`.
-uint32_t sort_passes_for(uint32_t cells)
-{
-    uint32_t bits = 0;
-    while (bits < 32 && (cells - 1u) >> bits) ++bits;
-    const uint32_t passes = (bits + 7u) / 8u;
-    return passes == 0 ? 1u : passes;
-}
-
 bool finite_half(uint32_t h16) { return ((h16 >> 10) & 0x1Fu) != 0x1Fu; }
 
 }  // namespace
@@ -137,18 +128,28 @@ struct mb_physics {
     uint32_t n = 0;
     int device = 0;
     int64_t time = -1;
+    // The reference's ping-pong buffers (mod.rs:37-51 and the sort's
+    // EVE/ODD keys/vals): frame t reads state[t & 1] and keys[t & 1] and
+    // writes state[!(t & 1)], keys[!(t & 1)] and the permutation vals[!(t & 1)].
     float4 *pos[2] = {nullptr, nullptr};
     uint4 *vfme[2] = {nullptr, nullptr};
     uint32_t *keys[2] = {nullptr, nullptr};
     uint32_t *vals[2] = {nullptr, nullptr};
-    void *sort_ws = nullptr;
-    // Profiling: four events per frame (before sort, before hashing, before
-    // physics, after physics) recorded without blocking; read back by
-    // mb_physics_profile_read.
+    uint32_t *off = nullptr;        // per-cell slot offsets, N + 1
+    uint32_t *rows = nullptr;       // row_count | row_fill | row_start
+    uint64_t *pairs = nullptr;      // N
+    uint64_t *scratch_pairs = nullptr;  // N
+    // Profiling: four events per frame (before the row partition, before the
+    // row sort, before collide + integrate, after it) recorded without
+    // blocking; read back by mb_physics_profile_read.
     bool profiling = false;
     std::vector<std::array<hipEvent_t, 4>> prof_events;
     size_t prof_used = 0;
     std::vector<double> prof_ms;  // [frame][3] of frames already read back
+
+    uint32_t *row_count() const { return rows; }
+    uint32_t *row_fill() const { return rows + mb::kMaxRows; }
+    uint32_t *row_start() const { return rows + 2 * mb::kMaxRows; }
 };
 
 extern "C" {
@@ -183,7 +184,10 @@ static void release(mb_physics *p)
         if (p->keys[i]) (void)hipFree(p->keys[i]);
         if (p->vals[i]) (void)hipFree(p->vals[i]);
     }
-    if (p->sort_ws) (void)hipFree(p->sort_ws);
+    if (p->off) (void)hipFree(p->off);
+    if (p->rows) (void)hipFree(p->rows);
+    if (p->pairs) (void)hipFree(p->pairs);
+    if (p->scratch_pairs) (void)hipFree(p->scratch_pairs);
     for (auto &fe : p->prof_events)
         for (hipEvent_t e : fe)
             if (e) (void)hipEventDestroy(e);
@@ -222,7 +226,7 @@ int mb_physics_create(const mb_settings *settings, mb_physics **out)
         return fail(MB_ERR_HIP, "no HIP device");
     }
     const size_t n = p->n;
-    const size_t ws = mb::sort_workspace_layout(p->n).bytes;
+    const size_t row_words = 3 * (size_t)mb::kMaxRows + 1;
     bool ok = true;
     for (int i = 0; i < 2 && ok; ++i) {
         ok = hipMalloc(&p->pos[i], n * sizeof(float4)) == hipSuccess &&
@@ -230,14 +234,18 @@ int mb_physics_create(const mb_settings *settings, mb_physics **out)
              hipMalloc(&p->keys[i], n * 4u) == hipSuccess &&
              hipMalloc(&p->vals[i], n * 4u) == hipSuccess;
     }
-    ok = ok && hipMalloc(&p->sort_ws, ws) == hipSuccess;
+    ok = ok && hipMalloc(&p->off, (n + 1) * 4u) == hipSuccess;
+    ok = ok && hipMalloc(&p->rows, row_words * 4u) == hipSuccess;
+    ok = ok && hipMalloc(&p->pairs, n * 8u) == hipSuccess;
+    ok = ok && hipMalloc(&p->scratch_pairs, n * 8u) == hipSuccess;
     if (!ok) {
         release(p);
         delete p;
         return fail(MB_ERR_OUT_OF_MEMORY, "device allocation failed");
     }
     // Buffers start zeroed, as wgpu's do (the INIT frame sorts all-zero keys).
-    ok = hipMemset(p->sort_ws, 0, ws) == hipSuccess;
+    ok = hipMemset(p->rows, 0, row_words * 4u) == hipSuccess &&
+         hipMemset(p->off, 0, (n + 1) * 4u) == hipSuccess;
     for (int i = 0; i < 2 && ok; ++i) {
         ok = hipMemset(p->pos[i], 0, n * sizeof(float4)) == hipSuccess &&
              hipMemset(p->vfme[i], 0, n * sizeof(uint4)) == hipSuccess &&
@@ -284,9 +292,13 @@ int mb_physics_set_settings(mb_physics *p, const mb_settings *s)
     p->settings = *s;
     p->consts = compile_consts(*s);
     if (regrid && p->time >= 0) {
+        // The next frame sorts keys of the new grid (the reference would sort
+        // the stale keys of the old grid once, reading stale `close` words).
         DeviceGuard g(p->device);
         const int cur = (int)((p->time + 1) & 1);
-        hipError_t e = mb::launch_rekey(p->consts, p->pos[cur], p->keys[cur], p->n, nullptr);
+        MB_HIP(hipMemset(p->rows, 0, 2 * (size_t)mb::kMaxRows * 4u));
+        hipError_t e = mb::launch_rekey(p->consts, p->pos[cur], p->keys[cur], p->row_count(),
+                                        p->n, nullptr);
         if (e != hipSuccess) return fail_hip(e, "rekey");
         MB_HIP(hipStreamSynchronize(nullptr));
     }
@@ -323,6 +335,7 @@ int mb_physics_upload_state_device(mb_physics *p, const void *pos, const void *v
         MB_HIP(hipMemsetAsync(p->keys[i], 0, n * 4u, st));
         MB_HIP(hipMemsetAsync(p->vals[i], 0, n * 4u, st));
     }
+    MB_HIP(hipMemsetAsync(p->rows, 0, (3 * (size_t)mb::kMaxRows + 1) * 4u, st));
     MB_HIP(hipStreamSynchronize(st));
     p->time = -1;
     return MB_OK;
@@ -345,7 +358,6 @@ static int run_frame(mb_physics *p, hipStream_t st)
     p->time += 1;
     const int par = (int)(p->time & 1);
     const int q = 1 - par;
-    const uint32_t cells = p->settings.number_of_grids_one_dimension * p->settings.number_of_grids_one_dimension;
     std::array<hipEvent_t, 4> *ev = nullptr;
     if (p->profiling) {
         constexpr size_t kRing = 4096;
@@ -359,22 +371,24 @@ static int run_frame(mb_physics *p, hipStream_t st)
             p->prof_events.push_back(fe);
         }
         ev = &p->prof_events[p->prof_used++];
-        MB_HIP(hipEventRecord((*ev)[0], st));
     }
-    // bevy_radix_sort::run(.., N, 0..3, true, is_even): keys[p] -> keys[q], vals[q]
-    hipError_t e = mb::launch_radix_sort(p->sort_ws, p->n, p->keys[par], nullptr, p->keys[q],
-                                         p->vals[q], p->n, 0u, sort_passes_for(cells), st);
-    if (e != hipSuccess) return fail_hip(e, "radix sort");
-    if (ev) MB_HIP(hipEventRecord((*ev)[1], st));
-    // spatial_hashing::run(.., N, !is_even): start -> keys[p], close -> vals[p]
-    e = mb::launch_spatial_hashing(p->keys[q], p->keys[par], p->vals[par], p->n, p->n, st);
-    if (e != hipSuccess) return fail_hip(e, "spatial hashing");
-    if (ev) MB_HIP(hipEventRecord((*ev)[2], st));
-    // physics::run: INIT at time 0, else LOOP (mod.rs:929-939)
-    e = mb::launch_physics(p->consts, p->time == 0, p->vals[q], p->keys[par], p->vals[par],
-                           p->keys[q], p->pos[par], p->vfme[par], p->pos[q], p->vfme[q], p->n, st);
-    if (e != hipSuccess) return fail_hip(e, "physics");
-    if (ev) MB_HIP(hipEventRecord((*ev)[3], st));
+    mb::FrameBuffers b;
+    b.keys_in = p->keys[par];
+    b.keys_out = p->keys[q];
+    b.perm = p->vals[q];
+    b.off = p->off;
+    b.row_count = p->row_count();
+    b.row_fill = p->row_fill();
+    b.row_start = p->row_start();
+    b.pairs = p->pairs;
+    b.scratch_pairs = p->scratch_pairs;
+    b.pos_i = p->pos[par];
+    b.vfme_i = p->vfme[par];
+    b.pos_o = p->pos[q];
+    b.vfme_o = p->vfme[q];
+    hipError_t e = mb::launch_frame(b, p->consts, p->time == 0, p->n, st,
+                                    ev ? ev->data() : nullptr);
+    if (e != hipSuccess) return fail_hip(e, "frame");
     return MB_OK;
 }
 
@@ -421,12 +435,25 @@ int mb_physics_debug_download(const mb_physics *p, uint32_t *perm, uint32_t *sta
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int par = (int)(p->time & 1);
     const int q = 1 - par;
-    const size_t bytes = (size_t)p->n * 4u;
-    if (perm) MB_HIP(hipMemcpyAsync(perm, p->vals[q], bytes, hipMemcpyDeviceToHost, st));
-    if (start) MB_HIP(hipMemcpyAsync(start, p->keys[par], bytes, hipMemcpyDeviceToHost, st));
-    if (close) MB_HIP(hipMemcpyAsync(close, p->vals[par], bytes, hipMemcpyDeviceToHost, st));
-    if (next_keys) MB_HIP(hipMemcpyAsync(next_keys, p->keys[q], bytes, hipMemcpyDeviceToHost, st));
+    const size_t n = p->n;
+    const size_t cells = (size_t)p->settings.number_of_grids_one_dimension *
+                         p->settings.number_of_grids_one_dimension;
+    std::vector<uint32_t> off;
+    if (start || close) {
+        off.resize(cells + 1);
+        MB_HIP(hipMemcpyAsync(off.data(), p->off, (cells + 1) * 4u, hipMemcpyDeviceToHost, st));
+    }
+    if (perm) MB_HIP(hipMemcpyAsync(perm, p->vals[q], n * 4u, hipMemcpyDeviceToHost, st));
+    if (next_keys) MB_HIP(hipMemcpyAsync(next_keys, p->keys[q], n * 4u, hipMemcpyDeviceToHost, st));
     MB_HIP(hipStreamSynchronize(st));
+    // start/close as spatial_hashing.wgsl leaves them: start is INVALID for
+    // empty cells (and for the entries past the last cell); close is defined
+    // only where start is valid (0 elsewhere).
+    for (size_t c = 0; (start || close) && c < n; ++c) {
+        const bool in = c < cells && off[c + 1] > off[c];
+        if (start) start[c] = in ? off[c] : MB_INVALID_INDEX;
+        if (close) close[c] = in ? off[c + 1] - 1u : 0u;
+    }
     return MB_OK;
 }
 

@@ -618,7 +618,66 @@ __device__ __forceinline__ void solve_cube(const PhysConsts &c, Agent &a)
     }
 }
 
-// main() with LOOP_PIPELINE (physics.wgsl:57-90) for sorted slot s.
+// verlet_integration (physics.wgsl:262-266) followed by `force = 0` (:78).
+__device__ __forceinline__ void verlet(const PhysConsts &c, Agent &a)
+{
+    const float ax = a.fx / a.m, ay = a.fy / a.m, az = a.fz / a.m;
+    a.vx = a.vx + ((c.gravity_x + ax) * c.t_delta) * c.t_delta;
+    a.vy = a.vy + ((c.gravity_y + ay) * c.t_delta) * c.t_delta;
+    a.vz = a.vz + ((c.gravity_z + az) * c.t_delta) * c.t_delta;
+    a.px = a.px + a.vx;
+    a.py = a.py + a.vy;
+    a.pz = a.pz + a.vz;
+    a.fx = 0.0f;
+    a.fy = 0.0f;
+    a.fz = 0.0f;
+}
+
+// The anti-stuck jitter closing handle_collisions (physics.wgsl:162-168);
+// hash3f1u is physics.wgsl:334-337.
+__device__ __forceinline__ void jitter(Agent &a, uint32_t s, float ox, float oy, float oz)
+{
+    const float dx = a.px - ox, dy = a.py - oy, dz = a.pz - oz;
+    if (sqrtf(dx * dx + dy * dy + dz * dz) > a.scale) {
+        const uint32_t v = s * (s ^ (s >> 15));
+        const float jx = ((float)(v * 0x1u) / 4294967296.0f) * 2.0f - 1.0f;
+        const float jy = ((float)(v * 0x1FFu) / 4294967296.0f) * 2.0f - 1.0f;
+        const float jz = ((float)(v * 0x3FFFFu) / 4294967296.0f) * 2.0f - 1.0f;
+        const float len = sqrtf(jx * jx + jy * jy + jz * jz);
+        a.px = a.px + jx / len;
+        a.py = a.py + jy / len;
+        a.pz = a.pz + jz / len;
+    }
+}
+
+// physics.wgsl:80-88 after the collisions: walls, f16 clamp, pack, store,
+// next cell key.  Returns the key.
+template <bool SPRING>
+__device__ __forceinline__ uint32_t finish_agent(const PhysConsts &c, Agent &a, uint32_t s,
+                                                 float4 *__restrict__ pos_o,
+                                                 uint4 *__restrict__ vfme_o,
+                                                 uint32_t *__restrict__ key_out)
+{
+    solve_cube<SPRING>(c, a);
+    a.vx = clamp_sel(a.vx, -65504.0f, 65504.0f);
+    a.vy = clamp_sel(a.vy, -65504.0f, 65504.0f);
+    a.vz = clamp_sel(a.vz, -65504.0f, 65504.0f);
+    a.fx = clamp_sel(a.fx, -65504.0f, 65504.0f);
+    a.fy = clamp_sel(a.fy, -65504.0f, 65504.0f);
+    a.fz = clamp_sel(a.fz, -65504.0f, 65504.0f);
+    pos_o[s] = make_float4(a.px, a.py, a.pz, a.scale);
+    vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
+                           pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
+    uint32_t nx, nz;
+    cell_of(c, a.px, a.pz, nx, nz);
+    const uint32_t key = nz * c.grids + nx;
+    key_out[s] = key;
+    return key;
+}
+
+// main() with LOOP_PIPELINE (physics.wgsl:57-90) for sorted slot s, reading
+// per-cell start/close arrays exactly as the shader does.  This is the
+// kernel behind the physics::run stage entry point (mb_physics_run).
 template <bool SPRING>
 __global__ __launch_bounds__(256) void physics_loop_kernel(
     PhysConsts c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ start,
@@ -633,22 +692,7 @@ __global__ __launch_bounds__(256) void physics_loop_kernel(
     uint32_t cx, cz;
     cell_of(c, a.px, a.pz, cx, cz);
     const uint32_t k = cz * c.grids + cx;
-
-    // verlet_integration (physics.wgsl:262-266)
-    {
-        const float ax = a.fx / a.m, ay = a.fy / a.m, az = a.fz / a.m;
-        a.vx = a.vx + ((c.gravity_x + ax) * c.t_delta) * c.t_delta;
-        a.vy = a.vy + ((c.gravity_y + ay) * c.t_delta) * c.t_delta;
-        a.vz = a.vz + ((c.gravity_z + az) * c.t_delta) * c.t_delta;
-        a.px = a.px + a.vx;
-        a.py = a.py + a.vy;
-        a.pz = a.pz + a.vz;
-        a.fx = 0.0f;
-        a.fy = 0.0f;
-        a.fz = 0.0f;
-    }
-
-    // handle_collisions (physics.wgsl:119-169)
+    verlet(c, a);
     const float ox = a.px, oy = a.py, oz = a.pz;
     {
         uint32_t first = start[k];
@@ -666,36 +710,8 @@ __global__ __launch_bounds__(256) void physics_loop_kernel(
     if (cz > 0u) handle_row<SPRING>(c, a, cx, k - c.grids, start, close, perm, pos_i, vfme_i);
     if (cz < c.grids - 1u)
         handle_row<SPRING>(c, a, cx, k + c.grids, start, close, perm, pos_i, vfme_i);
-    {
-        const float dx = a.px - ox, dy = a.py - oy, dz = a.pz - oz;
-        if (sqrtf(dx * dx + dy * dy + dz * dz) > a.scale) {
-            // hash3f1u (physics.wgsl:334-337)
-            const uint32_t v = s * (s ^ (s >> 15));
-            const float jx = ((float)(v * 0x1u) / 4294967296.0f) * 2.0f - 1.0f;
-            const float jy = ((float)(v * 0x1FFu) / 4294967296.0f) * 2.0f - 1.0f;
-            const float jz = ((float)(v * 0x3FFFFu) / 4294967296.0f) * 2.0f - 1.0f;
-            const float len = sqrtf(jx * jx + jy * jy + jz * jz);
-            a.px = a.px + jx / len;
-            a.py = a.py + jy / len;
-            a.pz = a.pz + jz / len;
-        }
-    }
-
-    solve_cube<SPRING>(c, a);
-
-    a.vx = clamp_sel(a.vx, -65504.0f, 65504.0f);
-    a.vy = clamp_sel(a.vy, -65504.0f, 65504.0f);
-    a.vz = clamp_sel(a.vz, -65504.0f, 65504.0f);
-    a.fx = clamp_sel(a.fx, -65504.0f, 65504.0f);
-    a.fy = clamp_sel(a.fy, -65504.0f, 65504.0f);
-    a.fz = clamp_sel(a.fz, -65504.0f, 65504.0f);
-
-    pos_o[s] = make_float4(a.px, a.py, a.pz, a.scale);
-    vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
-                           pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
-    uint32_t nx, nz;
-    cell_of(c, a.px, a.pz, nx, nz);
-    key_out[s] = nz * c.grids + nx;
+    jitter(a, s, ox, oy, oz);
+    (void)finish_agent<SPRING>(c, a, s, pos_o, vfme_o, key_out);
 }
 
 // main() with INIT_PIPELINE (physics.wgsl:91-96).
@@ -714,17 +730,6 @@ __global__ __launch_bounds__(256) void physics_init_kernel(
     pos_o[s] = p;
     vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
                            pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
-    key_out[s] = cz * c.grids + cx;
-}
-
-__global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *__restrict__ pos,
-                                                    uint32_t *__restrict__ key_out, uint32_t n)
-{
-    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
-    if (s >= n) return;
-    const float4 p = pos[s];
-    uint32_t cx, cz;
-    cell_of(c, p.x, p.z, cx, cz);
     key_out[s] = cz * c.grids + cx;
 }
 
@@ -748,12 +753,412 @@ hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
     return hipGetLastError();
 }
 
-hipError_t launch_rekey(const PhysConsts &c, const float4 *pos, uint32_t *key_out, uint32_t n,
-                        hipStream_t stream)
+// ===========================================================================
+// The frame path: row partition -> per-row sort -> staged collide + integrate
+// ===========================================================================
+//
+// A cell key is cz * G + cx, so the stable sort of the keys is the stable
+// partition of the spheres by row cz followed, inside each row, by the order
+// (cx, input index).  The frame path exploits that:
+//   row_partition_kernel  scatters (cx, index) pairs into their row's slot
+//                         range with atomics (order inside a row arbitrary);
+//   row_sort_kernel       one block per row: counting sort by cx in LDS, then
+//                         every pair is ranked among the pairs of its cell by
+//                         index, which restores argsort_stable exactly; it
+//                         also writes the per-cell slot offsets;
+//   physics_step_kernel   collide + integrate with the neighbour rows staged
+//                         in LDS; it writes the next keys and counts them per
+//                         row for the next frame's partition.
+// No stage needs a cross-block ordering, so there is no look-back and no spin.
+
+// Physical block b runs on XCD b % 8 (round-robin dispatch); give each XCD a
+// contiguous range of logical blocks so that neighbouring rows share an L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
+{
+    constexpr uint32_t kXcds = 8;
+    const uint32_t q = nb / kXcds, rem = nb % kXcds;
+    const uint32_t x = b % kXcds, j = b / kXcds;
+    return x * q + min(x, rem) + j;
+}
+
+// Adds `amount` lanes' worth to counter[key] once per distinct key in the
+// wave; returns this lane's slot (base + rank among the lanes sharing its key).
+// Lanes with active == false take no slot.
+__device__ __forceinline__ uint32_t wave_aggregated_add(uint32_t *counter, uint32_t key,
+                                                        bool active)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt = lanemask_lt(lane);
+    uint64_t todo = __ballot(active);
+    uint32_t slot = 0;
+    while (todo) {
+        const uint32_t leader = (uint32_t)__ffsll((long long)todo) - 1u;
+        const uint32_t lk = (uint32_t)__shfl((int)key, (int)leader, 64);
+        const uint64_t peers = __ballot(active && key == lk);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&counter[lk], (uint32_t)__popcll(peers));
+        base = (uint32_t)__shfl((int)base, (int)leader, 64);
+        if (active && key == lk) slot = base + (uint32_t)__popcll(peers & lt);
+        todo &= ~peers;
+    }
+    return slot;
+}
+
+// Exclusive scan, in place, of `count` (<= 4096) LDS words with a 256-thread
+// block.  Returns the total.  `scratch` is 4 LDS words.
+__device__ __forceinline__ uint32_t block256_scan_lds(uint32_t *vals, uint32_t count,
+                                                      uint32_t *scratch)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (count + 255u) / 256u;
+    const uint32_t beg = tid * per;
+    uint32_t sum = 0;
+    for (uint32_t j = 0; j < per; ++j)
+        if (beg + j < count) sum += vals[beg + j];
+    uint32_t total;
+    uint32_t run = block256_exclusive_scan(sum, scratch, total);
+    for (uint32_t j = 0; j < per; ++j) {
+        if (beg + j < count) {
+            const uint32_t v = vals[beg + j];
+            vals[beg + j] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    return total;
+}
+
+__global__ __launch_bounds__(256) void row_partition_kernel(
+    const uint32_t *__restrict__ keys, uint32_t n, uint32_t grids,
+    const uint32_t *__restrict__ row_count, uint32_t *__restrict__ row_fill,
+    uint32_t *__restrict__ row_start_out, uint64_t *__restrict__ pairs)
+{
+    __shared__ uint32_t s_start[kMaxRows + 1];
+    __shared__ uint32_t s_scratch[4];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t rows = grids;
+    for (uint32_t r = tid; r < rows; r += 256u) s_start[r] = row_count[r];
+    __syncthreads();
+    const uint32_t total = block256_scan_lds(s_start, rows, s_scratch);
+    if (tid == 0) s_start[rows] = total;
+    __syncthreads();
+    if (blockIdx.x == 0)
+        for (uint32_t r = tid; r <= rows; r += 256u) row_start_out[r] = s_start[r];
+
+    const uint32_t base = blockIdx.x * kPartTile;
+#pragma unroll 4
+    for (uint32_t j = 0; j < kPartItems; ++j) {
+        const uint32_t i = base + j * 256u + tid;
+        const bool ok = i < n;
+        const uint32_t k = ok ? keys[i] : 0u;
+        const uint32_t r = min(k / grids, rows - 1u);
+        const uint32_t cx = min(k - r * grids, grids - 1u);
+        const uint32_t slot = wave_aggregated_add(row_fill, r, ok);
+        if (ok) {
+            const uint32_t dest = s_start[r] + slot;
+            if (dest < s_start[r + 1u]) pairs[dest] = ((uint64_t)cx << 32) | i;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void row_sort_kernel(
+    const uint64_t *__restrict__ pairs, uint64_t *__restrict__ scratch_pairs,
+    const uint32_t *__restrict__ row_start, uint32_t *__restrict__ row_count,
+    uint32_t *__restrict__ row_fill, uint32_t grids, uint32_t n,
+    uint32_t *__restrict__ perm, uint32_t *__restrict__ off)
+{
+    __shared__ uint32_t s_cell[kMaxRows];
+    __shared__ uint64_t s_el[kRowCap];
+    __shared__ uint32_t s_scratch[4];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t r = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t base = row_start[r];
+    const uint32_t m = row_start[r + 1u] - base;
+    for (uint32_t c = tid; c < grids; c += 256u) s_cell[c] = 0u;
+    __syncthreads();
+
+    if (m <= kRowCap) {
+        constexpr uint32_t kPer = kRowCap / 256u;
+        uint64_t el[kPer];
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) {
+            const uint32_t e = j * 256u + tid;
+            el[j] = e < m ? pairs[base + e] : 0ull;
+            if (e < m) atomicAdd(&s_cell[(uint32_t)(el[j] >> 32)], 1u);
+        }
+        __syncthreads();
+        (void)block256_scan_lds(s_cell, grids, s_scratch);
+        for (uint32_t c = tid; c < grids; c += 256u) off[r * grids + c] = base + s_cell[c];
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) {
+            const uint32_t e = j * 256u + tid;
+            if (e < m) s_el[atomicAdd(&s_cell[(uint32_t)(el[j] >> 32)], 1u)] = el[j];
+        }
+        __syncthreads();
+        // s_cell[c] is now the end of cell c inside the row.
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) {
+            const uint32_t e = j * 256u + tid;
+            if (e < m) {
+                const uint32_t c = (uint32_t)(el[j] >> 32);
+                const uint32_t idx = (uint32_t)el[j];
+                const uint32_t cs = c ? s_cell[c - 1u] : 0u;
+                const uint32_t ce = s_cell[c];
+                uint32_t rank = 0;
+                for (uint32_t q = cs; q < ce; ++q) rank += (uint32_t)s_el[q] < idx;
+                perm[base + cs + rank] = idx;
+            }
+        }
+    } else {
+        // A row larger than the LDS tile: the same steps with the scattered
+        // copy in global scratch.
+        for (uint32_t e = tid; e < m; e += 256u)
+            atomicAdd(&s_cell[(uint32_t)(pairs[base + e] >> 32)], 1u);
+        __syncthreads();
+        (void)block256_scan_lds(s_cell, grids, s_scratch);
+        for (uint32_t c = tid; c < grids; c += 256u) off[r * grids + c] = base + s_cell[c];
+        __syncthreads();
+        for (uint32_t e = tid; e < m; e += 256u) {
+            const uint64_t v = pairs[base + e];
+            scratch_pairs[base + atomicAdd(&s_cell[(uint32_t)(v >> 32)], 1u)] = v;
+        }
+        __threadfence_block();
+        __syncthreads();
+        for (uint32_t e = tid; e < m; e += 256u) {
+            const uint64_t v = pairs[base + e];
+            const uint32_t c = (uint32_t)(v >> 32);
+            const uint32_t idx = (uint32_t)v;
+            const uint32_t cs = c ? s_cell[c - 1u] : 0u;
+            const uint32_t ce = s_cell[c];
+            uint32_t rank = 0;
+            for (uint32_t q = cs; q < ce; ++q) rank += (uint32_t)scratch_pairs[base + q] < idx;
+            perm[base + cs + rank] = idx;
+        }
+    }
+    if (tid == 0) {
+        row_count[r] = 0u;
+        row_fill[r] = 0u;
+        if (r == grids - 1u) off[grids * grids] = n;
+    }
+}
+
+// Candidate source: LDS window w if slot i lies in it, else global memory.
+struct StageWindows {
+    uint32_t lo[3];
+    uint32_t cnt[3];
+};
+
+template <bool SPRING>
+__device__ __forceinline__ void collide_candidate(const PhysConsts &c, Agent &a, float4 bp,
+                                                  uint32_t bo, const uint4 *__restrict__ vfme_i)
+{
+    // Cheap conservative filter: sqrt_rn(d2) <= edge implies
+    // d2 <= edge^2 (1 + 2^-23), far below the 1e-4 margin used here; the exact
+    // test of solve_sphere_constraint runs for every candidate that passes.
+    const float edge = bp.w + a.scale;
+    const float tx = a.px - bp.x;
+    const float ty = a.py - bp.y;
+    const float tz = a.pz - bp.z;
+    const float d2 = tx * tx + ty * ty + tz * tz;
+    if (d2 <= (edge * edge) * 1.0001f) solve_sphere<SPRING>(c, a, bp, vfme_i, bo);
+}
+
+// main() with LOOP_PIPELINE for a block of 256 consecutive sorted slots.
+// The slots' cells are non-decreasing, so the candidates of the whole block
+// in each of the three rows form one contiguous slot window; the windows'
+// positions (and original indices) are staged in LDS once, and each thread
+// walks its own three ranges (physics.wgsl:119-206) in the shader's order.
+template <bool SPRING>
+__global__ __launch_bounds__(256) void physics_step_kernel(
+    PhysConsts c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
+    uint32_t *__restrict__ key_out, uint32_t *__restrict__ row_count,
+    const float4 *__restrict__ pos_i, const uint4 *__restrict__ vfme_i,
+    float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o, uint32_t n)
+{
+    __shared__ float4 s_pos[3][kStageCap];
+    __shared__ uint32_t s_idx[3][kStageCap];
+    __shared__ StageWindows s_win;
+    __shared__ uint32_t s_kfirst, s_klast;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t s0 = blk * 256u;
+    const uint32_t last = min(n - s0, 256u) - 1u;
+    const uint32_t s = s0 + min(tid, last);  // idle threads mirror the last slot
+    const bool valid = tid <= last;
+    const uint32_t G = c.grids;
+    const uint32_t C = G * G;
+
+    const uint32_t o = perm[s];
+    Agent a = agent_from(pos_i[o], vfme_i[o]);
+    uint32_t cx, cz;
+    cell_of(c, a.px, a.pz, cx, cz);
+    const uint32_t k = cz * G + cx;
+    if (tid == 0) s_kfirst = k;
+    if (tid == last) s_klast = k;
+    __syncthreads();
+    if (tid < 3u) {
+        const int64_t dk = tid == 0 ? 0 : (tid == 1 ? -(int64_t)G : (int64_t)G);
+        int64_t lc = (int64_t)s_kfirst + dk - 1;
+        int64_t hc = (int64_t)s_klast + dk + 2;
+        lc = lc < 0 ? 0 : (lc > (int64_t)C ? (int64_t)C : lc);
+        hc = hc < 0 ? 0 : (hc > (int64_t)C ? (int64_t)C : hc);
+        const uint32_t lo = off[lc];
+        const uint32_t hi = max(off[hc], lo);
+        s_win.lo[tid] = lo;
+        s_win.cnt[tid] = min(hi - lo, kStageCap);
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t w = 0; w < 3; ++w) {
+        const uint32_t lo = s_win.lo[w], cnt = s_win.cnt[w];
+        for (uint32_t j = tid; j < cnt; j += 256u) {
+            const uint32_t bo = perm[lo + j];
+            s_idx[w][j] = bo;
+            s_pos[w][j] = pos_i[bo];
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+
+    // ranges of physics.wgsl:123-160 as half-open slot intervals
+    const uint32_t lft = cx > 0u ? 1u : 0u;
+    const uint32_t rgt = cx < G - 1u ? 2u : 1u;
+    uint32_t lo0 = off[k - lft], hi0 = off[k + rgt];
+    uint32_t lo1 = 0, hi1 = 0, lo2 = 0, hi2 = 0;
+    if (cz > 0u) {
+        lo1 = off[k - G - lft];
+        hi1 = off[k - G + rgt];
+    }
+    if (cz < G - 1u) {
+        lo2 = off[k + G - lft];
+        hi2 = off[k + G + rgt];
+    }
+    const uint32_t n0 = hi0 - lo0, n1 = hi1 - lo1, n2 = hi2 - lo2;
+    const uint32_t total = n0 + n1 + n2;
+
+    verlet(c, a);
+    const float ox = a.px, oy = a.py, oz = a.pz;
+    for (uint32_t j = 0; j < total; ++j) {
+        uint32_t i, w;
+        if (j < n0) {
+            i = lo0 + j;
+            w = 0;
+        } else if (j < n0 + n1) {
+            i = lo1 + (j - n0);
+            w = 1;
+        } else {
+            i = lo2 + (j - n0 - n1);
+            w = 2;
+        }
+        if (i == s) continue;  // only possible in the middle row
+        const uint32_t rel = i - s_win.lo[w];
+        float4 bp;
+        uint32_t bo;
+        if (rel < s_win.cnt[w]) {
+            bp = s_pos[w][rel];
+            bo = s_idx[w][rel];
+        } else {
+            bo = perm[i];
+            bp = pos_i[bo];
+        }
+        collide_candidate<SPRING>(c, a, bp, bo, vfme_i);
+    }
+    jitter(a, s, ox, oy, oz);
+    const uint32_t key = finish_agent<SPRING>(c, a, s, pos_o, vfme_o, key_out);
+    const uint32_t row = key / G;
+    (void)wave_aggregated_add(row_count, row, true);
+}
+
+// INIT frame of the frame path: the sort of the all-zero keys is the
+// identity, so perm[s] = s and every sphere sits in cell 0's range.
+__global__ __launch_bounds__(256) void physics_init_step_kernel(
+    PhysConsts c, uint32_t *__restrict__ perm, uint32_t *__restrict__ key_out,
+    uint32_t *__restrict__ row_count, const float4 *__restrict__ pos_i,
+    const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
+    uint32_t n)
+{
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= n) return;
+    perm[s] = s;
+    const float4 p = pos_i[s];
+    const Agent a = agent_from(p, vfme_i[s]);
+    uint32_t cx, cz;
+    cell_of(c, a.px, a.pz, cx, cz);
+    pos_o[s] = p;
+    vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
+                           pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
+    key_out[s] = cz * c.grids + cx;
+    (void)wave_aggregated_add(row_count, cz, true);
+}
+
+// Offsets of the INIT frame: cell 0 holds every sphere.
+__global__ __launch_bounds__(256) void init_offsets_kernel(uint32_t *__restrict__ off, uint32_t cells,
+                                                           uint32_t n)
+{
+    const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+    if (c <= cells) off[c] = c == 0 ? 0u : n;
+}
+
+__global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *__restrict__ pos,
+                                                    uint32_t *__restrict__ key_out,
+                                                    uint32_t *__restrict__ row_count, uint32_t n)
+{
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= n) return;
+    const float4 p = pos[s];
+    uint32_t cx, cz;
+    cell_of(c, p.x, p.z, cx, cz);
+    key_out[s] = cz * c.grids + cx;
+    (void)wave_aggregated_add(row_count, cz, true);
+}
+
+hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, bool init, uint32_t n,
+                        hipStream_t stream, hipEvent_t *events)
+{
+    if (n == 0) return hipSuccess;
+    const uint32_t G = c.grids;
+    const uint32_t cells = G * G;
+    const dim3 grid((n + 255u) / 256u);
+    auto mark = [&](int i) {
+        return events ? hipEventRecord(events[i], stream) : hipSuccess;
+    };
+    hipError_t e = mark(0);
+    if (e != hipSuccess) return e;
+    if (init) {
+        hipLaunchKernelGGL(init_offsets_kernel, dim3((cells + 256u) / 256u), dim3(256), 0, stream,
+                           b.off, cells, n);
+        if ((e = mark(1)) != hipSuccess || (e = mark(2)) != hipSuccess) return e;
+        hipLaunchKernelGGL(physics_init_step_kernel, grid, dim3(256), 0, stream, c, b.perm,
+                           b.keys_out, b.row_count, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, n);
+        if ((e = mark(3)) != hipSuccess) return e;
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(row_partition_kernel, dim3((n + kPartTile - 1u) / kPartTile), dim3(256), 0,
+                       stream, b.keys_in, n, G, b.row_count, b.row_fill, b.row_start, b.pairs);
+    if ((e = mark(1)) != hipSuccess) return e;
+    hipLaunchKernelGGL(row_sort_kernel, dim3(G), dim3(256), 0, stream, b.pairs, b.scratch_pairs,
+                       b.row_start, b.row_count, b.row_fill, G, n, b.perm, b.off);
+    if ((e = mark(2)) != hipSuccess) return e;
+    if (c.enable_spring) {
+        hipLaunchKernelGGL(physics_step_kernel<true>, grid, dim3(256), 0, stream, c, b.perm, b.off,
+                           b.keys_out, b.row_count, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, n);
+    } else {
+        hipLaunchKernelGGL(physics_step_kernel<false>, grid, dim3(256), 0, stream, c, b.perm,
+                           b.off, b.keys_out, b.row_count, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o,
+                           n);
+    }
+    if ((e = mark(3)) != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+hipError_t launch_rekey(const PhysConsts &c, const float4 *pos, uint32_t *key_out,
+                        uint32_t *row_count, uint32_t n, hipStream_t stream)
 {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(rekey_kernel, dim3((n + 255u) / 256u), dim3(256), 0, stream, c, pos,
-                       key_out, n);
+                       key_out, row_count, n);
     return hipGetLastError();
 }
 
