@@ -3,9 +3,9 @@
 Workload (default): 1,048,576 spheres, 1024 x 1024 cells of size 1, half map
 height 8, radius 0.5, dt 1/60, e_agent = e_envir = 0.8, spring off, seeded
 uniform positions inside the box and velocities U[-1,1]^3 (per-step units).
-A "step" is one LOOP frame (radix sort -> spatial hashing -> collide +
-integrate) of that scene with its state resident in HBM; the INIT frame and
-the warmup frames run before the timed region.
+A "step" is one LOOP frame (stable sort of the cell keys with the per-cell
+ranges, then collide + integrate) of that scene with its state resident in
+HBM; the INIT frame and the warmup frames run before the timed region.
 
 Prints one JSON line (rank 0).  `roofline` is computed for the collide +
 integrate kernel from its HIP-event durations over a profiled run of the same
@@ -177,7 +177,7 @@ def main() -> int:
         "stage_achieved_GBs": {"row_partition": part_gbs, "row_sort": rsort_gbs,
                                "collide_integrate": achieved},
         "roofline": {
-            "kernel": "physics_loop_kernel (collide + integrate)",
+            "kernel": "physics_step_kernel (collide + integrate)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

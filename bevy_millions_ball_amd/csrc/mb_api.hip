@@ -139,6 +139,8 @@ struct mb_physics {
     uint32_t *rows = nullptr;       // row_count | row_fill | row_start
     uint64_t *pairs = nullptr;      // N
     uint64_t *scratch_pairs = nullptr;  // N
+    float4 *spos = nullptr;         // N: input state gathered into sorted order
+    uint4 *svfme = nullptr;         // N
     // Profiling: four events per frame (before the row partition, before the
     // row sort, before collide + integrate, after it) recorded without
     // blocking; read back by mb_physics_profile_read.
@@ -188,6 +190,8 @@ static void release(mb_physics *p)
     if (p->rows) (void)hipFree(p->rows);
     if (p->pairs) (void)hipFree(p->pairs);
     if (p->scratch_pairs) (void)hipFree(p->scratch_pairs);
+    if (p->spos) (void)hipFree(p->spos);
+    if (p->svfme) (void)hipFree(p->svfme);
     for (auto &fe : p->prof_events)
         for (hipEvent_t e : fe)
             if (e) (void)hipEventDestroy(e);
@@ -238,6 +242,8 @@ int mb_physics_create(const mb_settings *settings, mb_physics **out)
     ok = ok && hipMalloc(&p->rows, row_words * 4u) == hipSuccess;
     ok = ok && hipMalloc(&p->pairs, n * 8u) == hipSuccess;
     ok = ok && hipMalloc(&p->scratch_pairs, n * 8u) == hipSuccess;
+    ok = ok && hipMalloc(&p->spos, n * sizeof(float4)) == hipSuccess;
+    ok = ok && hipMalloc(&p->svfme, n * sizeof(uint4)) == hipSuccess;
     if (!ok) {
         release(p);
         delete p;
@@ -382,6 +388,8 @@ static int run_frame(mb_physics *p, hipStream_t st)
     b.row_start = p->row_start();
     b.pairs = p->pairs;
     b.scratch_pairs = p->scratch_pairs;
+    b.spos = p->spos;
+    b.svfme = p->svfme;
     b.pos_i = p->pos[par];
     b.vfme_i = p->vfme[par];
     b.pos_o = p->pos[q];

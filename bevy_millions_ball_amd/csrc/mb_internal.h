@@ -78,6 +78,8 @@ struct FrameBuffers {
     uint32_t *row_start;       // kMaxRows + 1: exclusive scan of row_count
     uint64_t *pairs;           // n: (cx, index) partitioned by row
     uint64_t *scratch_pairs;   // n: rows larger than kRowCap
+    float4 *spos;              // n: input positions gathered into sorted order
+    uint4 *svfme;              // n: input packed_vfme gathered into sorted order
     const float4 *pos_i;
     const uint4 *vfme_i;
     float4 *pos_o;

@@ -761,15 +761,18 @@ hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
 // partition of the spheres by row cz followed, inside each row, by the order
 // (cx, input index).  The frame path exploits that:
 //   row_partition_kernel  scatters (cx, index) pairs into their row's slot
-//                         range with atomics (order inside a row arbitrary);
+//                         range; each block reserves one sub-range per row it
+//                         touches (order inside a row is arbitrary);
 //   row_sort_kernel       one block per row: counting sort by cx in LDS, then
 //                         every pair is ranked among the pairs of its cell by
 //                         index, which restores argsort_stable exactly; it
-//                         also writes the per-cell slot offsets;
-//   physics_step_kernel   collide + integrate with the neighbour rows staged
-//                         in LDS; it writes the next keys and counts them per
-//                         row for the next frame's partition.
-// No stage needs a cross-block ordering, so there is no look-back and no spin.
+//                         writes perm, the per-cell slot offsets and the input
+//                         state gathered into sorted order;
+//   physics_step_kernel   collide + integrate on the sorted state with the
+//                         neighbour-row windows staged in LDS; it writes the
+//                         next keys and adds them to the next frame's row
+//                         histogram.
+// No stage needs a cross-block ordering: there is no look-back and no spin.
 
 // Physical block b runs on XCD b % 8 (round-robin dispatch); give each XCD a
 // contiguous range of logical blocks so that neighbouring rows share an L2.
@@ -781,11 +784,10 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
     return x * q + min(x, rem) + j;
 }
 
-// Adds `amount` lanes' worth to counter[key] once per distinct key in the
-// wave; returns this lane's slot (base + rank among the lanes sharing its key).
+// LDS counter[key] += lanes sharing `key`, one atomic per distinct key of
+// the wave; returns this lane's slot (old value + rank among those lanes).
 // Lanes with active == false take no slot.
-__device__ __forceinline__ uint32_t wave_aggregated_add(uint32_t *counter, uint32_t key,
-                                                        bool active)
+__device__ __forceinline__ uint32_t wave_slot_lds(uint32_t *counter, uint32_t key, bool active)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = lanemask_lt(lane);
@@ -794,14 +796,30 @@ __device__ __forceinline__ uint32_t wave_aggregated_add(uint32_t *counter, uint3
     while (todo) {
         const uint32_t leader = (uint32_t)__ffsll((long long)todo) - 1u;
         const uint32_t lk = (uint32_t)__shfl((int)key, (int)leader, 64);
-        const uint64_t peers = __ballot(active && key == lk);
+        const bool mine = active && key == lk;
+        const uint64_t peers = __ballot(mine);
         uint32_t base = 0;
         if (lane == leader) base = atomicAdd(&counter[lk], (uint32_t)__popcll(peers));
         base = (uint32_t)__shfl((int)base, (int)leader, 64);
-        if (active && key == lk) slot = base + (uint32_t)__popcll(peers & lt);
+        if (mine) slot = base + (uint32_t)__popcll(peers & lt);
         todo &= ~peers;
     }
     return slot;
+}
+
+// Global counter[key] += 1 for every active lane, one non-returning atomic
+// per distinct key of the wave.
+__device__ __forceinline__ void wave_count_global(uint32_t *counter, uint32_t key, bool active)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    uint64_t todo = __ballot(active);
+    while (todo) {
+        const uint32_t leader = (uint32_t)__ffsll((long long)todo) - 1u;
+        const uint32_t lk = (uint32_t)__shfl((int)key, (int)leader, 64);
+        const uint64_t peers = __ballot(active && key == lk);
+        if (lane == leader) atomicAdd(&counter[lk], (uint32_t)__popcll(peers));
+        todo &= ~peers;
+    }
 }
 
 // Exclusive scan, in place, of `count` (<= 4096) LDS words with a 256-thread
@@ -834,10 +852,14 @@ __global__ __launch_bounds__(256) void row_partition_kernel(
     uint32_t *__restrict__ row_start_out, uint64_t *__restrict__ pairs)
 {
     __shared__ uint32_t s_start[kMaxRows + 1];
+    __shared__ uint32_t s_mine[kMaxRows];  // this block's count, then its base, per row
     __shared__ uint32_t s_scratch[4];
     const uint32_t tid = threadIdx.x;
     const uint32_t rows = grids;
-    for (uint32_t r = tid; r < rows; r += 256u) s_start[r] = row_count[r];
+    for (uint32_t r = tid; r < rows; r += 256u) {
+        s_start[r] = row_count[r];
+        s_mine[r] = 0u;
+    }
     __syncthreads();
     const uint32_t total = block256_scan_lds(s_start, rows, s_scratch);
     if (tid == 0) s_start[rows] = total;
@@ -846,16 +868,33 @@ __global__ __launch_bounds__(256) void row_partition_kernel(
         for (uint32_t r = tid; r <= rows; r += 256u) row_start_out[r] = s_start[r];
 
     const uint32_t base = blockIdx.x * kPartTile;
-#pragma unroll 4
+    uint32_t key[kPartItems];
+    uint32_t slot[kPartItems];
+#pragma unroll
     for (uint32_t j = 0; j < kPartItems; ++j) {
         const uint32_t i = base + j * 256u + tid;
-        const bool ok = i < n;
-        const uint32_t k = ok ? keys[i] : 0u;
-        const uint32_t r = min(k / grids, rows - 1u);
-        const uint32_t cx = min(k - r * grids, grids - 1u);
-        const uint32_t slot = wave_aggregated_add(row_fill, r, ok);
-        if (ok) {
-            const uint32_t dest = s_start[r] + slot;
+        key[j] = i < n ? keys[i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kPartItems; ++j) {
+        const uint32_t i = base + j * 256u + tid;
+        const uint32_t r = min(key[j] / grids, rows - 1u);
+        slot[j] = wave_slot_lds(s_mine, r, i < n);
+    }
+    __syncthreads();
+    // One reservation per row this block touches.
+    for (uint32_t r = tid; r < rows; r += 256u) {
+        const uint32_t c = s_mine[r];
+        if (c) s_mine[r] = s_start[r] + atomicAdd(&row_fill[r], c);
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < kPartItems; ++j) {
+        const uint32_t i = base + j * 256u + tid;
+        if (i < n) {
+            const uint32_t r = min(key[j] / grids, rows - 1u);
+            const uint32_t cx = min(key[j] - r * grids, grids - 1u);
+            const uint32_t dest = s_mine[r] + slot[j];
             if (dest < s_start[r + 1u]) pairs[dest] = ((uint64_t)cx << 32) | i;
         }
     }
@@ -865,7 +904,9 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
     const uint64_t *__restrict__ pairs, uint64_t *__restrict__ scratch_pairs,
     const uint32_t *__restrict__ row_start, uint32_t *__restrict__ row_count,
     uint32_t *__restrict__ row_fill, uint32_t grids, uint32_t n,
-    uint32_t *__restrict__ perm, uint32_t *__restrict__ off)
+    uint32_t *__restrict__ perm, uint32_t *__restrict__ off,
+    const float4 *__restrict__ pos_i, const uint4 *__restrict__ vfme_i,
+    float4 *__restrict__ spos, uint4 *__restrict__ svfme)
 {
     __shared__ uint32_t s_cell[kMaxRows];
     __shared__ uint64_t s_el[kRowCap];
@@ -884,6 +925,10 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
         for (uint32_t j = 0; j < kPer; ++j) {
             const uint32_t e = j * 256u + tid;
             el[j] = e < m ? pairs[base + e] : 0ull;
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) {
+            const uint32_t e = j * 256u + tid;
             if (e < m) atomicAdd(&s_cell[(uint32_t)(el[j] >> 32)], 1u);
         }
         __syncthreads();
@@ -896,10 +941,13 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
             if (e < m) s_el[atomicAdd(&s_cell[(uint32_t)(el[j] >> 32)], 1u)] = el[j];
         }
         __syncthreads();
-        // s_cell[c] is now the end of cell c inside the row.
+        // s_cell[c] is now the end of cell c inside the row; rank each pair
+        // among its cell's pairs by input index.
+        uint32_t dst[kPer];
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) {
             const uint32_t e = j * 256u + tid;
+            dst[j] = kInvalid;
             if (e < m) {
                 const uint32_t c = (uint32_t)(el[j] >> 32);
                 const uint32_t idx = (uint32_t)el[j];
@@ -907,8 +955,20 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
                 const uint32_t ce = s_cell[c];
                 uint32_t rank = 0;
                 for (uint32_t q = cs; q < ce; ++q) rank += (uint32_t)s_el[q] < idx;
-                perm[base + cs + rank] = idx;
+                dst[j] = cs + rank;
             }
+        }
+        __syncthreads();
+        uint32_t *s_perm = reinterpret_cast<uint32_t *>(s_el);
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j)
+            if (dst[j] != kInvalid) s_perm[dst[j]] = (uint32_t)el[j];
+        __syncthreads();
+        for (uint32_t e = tid; e < m; e += 256u) {
+            const uint32_t idx = s_perm[e];
+            perm[base + e] = idx;
+            spos[base + e] = pos_i[idx];
+            svfme[base + e] = vfme_i[idx];
         }
     } else {
         // A row larger than the LDS tile: the same steps with the scattered
@@ -933,7 +993,10 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
             const uint32_t ce = s_cell[c];
             uint32_t rank = 0;
             for (uint32_t q = cs; q < ce; ++q) rank += (uint32_t)scratch_pairs[base + q] < idx;
-            perm[base + cs + rank] = idx;
+            const uint32_t slot = base + cs + rank;
+            perm[slot] = idx;
+            spos[slot] = pos_i[idx];
+            svfme[slot] = vfme_i[idx];
         }
     }
     if (tid == 0) {
@@ -943,7 +1006,6 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
     }
 }
 
-// Candidate source: LDS window w if slot i lies in it, else global memory.
 struct StageWindows {
     uint32_t lo[3];
     uint32_t cnt[3];
@@ -951,33 +1013,34 @@ struct StageWindows {
 
 template <bool SPRING>
 __device__ __forceinline__ void collide_candidate(const PhysConsts &c, Agent &a, float4 bp,
-                                                  uint32_t bo, const uint4 *__restrict__ vfme_i)
+                                                  const uint4 *__restrict__ svfme, uint32_t i)
 {
     // Cheap conservative filter: sqrt_rn(d2) <= edge implies
-    // d2 <= edge^2 (1 + 2^-23), far below the 1e-4 margin used here; the exact
-    // test of solve_sphere_constraint runs for every candidate that passes.
+    // d2 <= edge^2 (1 + 2^-23), far inside the 1e-4 margin used here; the
+    // exact test of solve_sphere_constraint runs for every candidate that
+    // passes.
     const float edge = bp.w + a.scale;
     const float tx = a.px - bp.x;
     const float ty = a.py - bp.y;
     const float tz = a.pz - bp.z;
     const float d2 = tx * tx + ty * ty + tz * tz;
-    if (d2 <= (edge * edge) * 1.0001f) solve_sphere<SPRING>(c, a, bp, vfme_i, bo);
+    if (d2 <= (edge * edge) * 1.0001f) solve_sphere<SPRING>(c, a, bp, svfme, i);
 }
 
-// main() with LOOP_PIPELINE for a block of 256 consecutive sorted slots.
-// The slots' cells are non-decreasing, so the candidates of the whole block
-// in each of the three rows form one contiguous slot window; the windows'
-// positions (and original indices) are staged in LDS once, and each thread
-// walks its own three ranges (physics.wgsl:119-206) in the shader's order.
+// main() with LOOP_PIPELINE for a block of 256 consecutive sorted slots, on
+// the state already gathered into sorted order.  The slots' cells are
+// non-decreasing, so the candidates of the whole block in each of the three
+// rows form one contiguous slot window; the windows are copied to LDS once,
+// and each thread walks its own three ranges (physics.wgsl:119-206) in the
+// shader's order.
 template <bool SPRING>
 __global__ __launch_bounds__(256) void physics_step_kernel(
-    PhysConsts c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
-    uint32_t *__restrict__ key_out, uint32_t *__restrict__ row_count,
-    const float4 *__restrict__ pos_i, const uint4 *__restrict__ vfme_i,
-    float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o, uint32_t n)
+    PhysConsts c, const uint32_t *__restrict__ off, uint32_t *__restrict__ key_out,
+    uint32_t *__restrict__ row_count, const float4 *__restrict__ spos,
+    const uint4 *__restrict__ svfme, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
+    uint32_t n)
 {
     __shared__ float4 s_pos[3][kStageCap];
-    __shared__ uint32_t s_idx[3][kStageCap];
     __shared__ StageWindows s_win;
     __shared__ uint32_t s_kfirst, s_klast;
 
@@ -990,13 +1053,26 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     const uint32_t G = c.grids;
     const uint32_t C = G * G;
 
-    const uint32_t o = perm[s];
-    Agent a = agent_from(pos_i[o], vfme_i[o]);
+    Agent a = agent_from(spos[s], svfme[s]);
     uint32_t cx, cz;
     cell_of(c, a.px, a.pz, cx, cz);
     const uint32_t k = cz * G + cx;
     if (tid == 0) s_kfirst = k;
     if (tid == last) s_klast = k;
+
+    // own ranges of physics.wgsl:123-160 as half-open slot intervals
+    const uint32_t lft = cx > 0u ? 1u : 0u;
+    const uint32_t rgt = cx < G - 1u ? 2u : 1u;
+    const uint32_t lo0 = off[k - lft], hi0 = off[k + rgt];
+    uint32_t lo1 = 0, hi1 = 0, lo2 = 0, hi2 = 0;
+    if (cz > 0u) {
+        lo1 = off[k - G - lft];
+        hi1 = off[k - G + rgt];
+    }
+    if (cz < G - 1u) {
+        lo2 = off[k + G - lft];
+        hi2 = off[k + G + rgt];
+    }
     __syncthreads();
     if (tid < 3u) {
         const int64_t dk = tid == 0 ? 0 : (tid == 1 ? -(int64_t)G : (int64_t)G);
@@ -1013,31 +1089,13 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
 #pragma unroll
     for (uint32_t w = 0; w < 3; ++w) {
         const uint32_t lo = s_win.lo[w], cnt = s_win.cnt[w];
-        for (uint32_t j = tid; j < cnt; j += 256u) {
-            const uint32_t bo = perm[lo + j];
-            s_idx[w][j] = bo;
-            s_pos[w][j] = pos_i[bo];
-        }
+        for (uint32_t j = tid; j < cnt; j += 256u) s_pos[w][j] = spos[lo + j];
     }
     __syncthreads();
     if (!valid) return;
 
-    // ranges of physics.wgsl:123-160 as half-open slot intervals
-    const uint32_t lft = cx > 0u ? 1u : 0u;
-    const uint32_t rgt = cx < G - 1u ? 2u : 1u;
-    uint32_t lo0 = off[k - lft], hi0 = off[k + rgt];
-    uint32_t lo1 = 0, hi1 = 0, lo2 = 0, hi2 = 0;
-    if (cz > 0u) {
-        lo1 = off[k - G - lft];
-        hi1 = off[k - G + rgt];
-    }
-    if (cz < G - 1u) {
-        lo2 = off[k + G - lft];
-        hi2 = off[k + G + rgt];
-    }
     const uint32_t n0 = hi0 - lo0, n1 = hi1 - lo1, n2 = hi2 - lo2;
     const uint32_t total = n0 + n1 + n2;
-
     verlet(c, a);
     const float ox = a.px, oy = a.py, oz = a.pz;
     for (uint32_t j = 0; j < total; ++j) {
@@ -1054,21 +1112,12 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
         }
         if (i == s) continue;  // only possible in the middle row
         const uint32_t rel = i - s_win.lo[w];
-        float4 bp;
-        uint32_t bo;
-        if (rel < s_win.cnt[w]) {
-            bp = s_pos[w][rel];
-            bo = s_idx[w][rel];
-        } else {
-            bo = perm[i];
-            bp = pos_i[bo];
-        }
-        collide_candidate<SPRING>(c, a, bp, bo, vfme_i);
+        const float4 bp = rel < s_win.cnt[w] ? s_pos[w][rel] : spos[i];
+        collide_candidate<SPRING>(c, a, bp, svfme, i);
     }
     jitter(a, s, ox, oy, oz);
     const uint32_t key = finish_agent<SPRING>(c, a, s, pos_o, vfme_o, key_out);
-    const uint32_t row = key / G;
-    (void)wave_aggregated_add(row_count, row, true);
+    wave_count_global(row_count, key / G, true);
 }
 
 // INIT frame of the frame path: the sort of the all-zero keys is the
@@ -1090,7 +1139,7 @@ __global__ __launch_bounds__(256) void physics_init_step_kernel(
     vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
                            pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
     key_out[s] = cz * c.grids + cx;
-    (void)wave_aggregated_add(row_count, cz, true);
+    wave_count_global(row_count, cz, true);
 }
 
 // Offsets of the INIT frame: cell 0 holds every sphere.
@@ -1111,7 +1160,7 @@ __global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *
     uint32_t cx, cz;
     cell_of(c, p.x, p.z, cx, cz);
     key_out[s] = cz * c.grids + cx;
-    (void)wave_aggregated_add(row_count, cz, true);
+    wave_count_global(row_count, cz, true);
 }
 
 hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, bool init, uint32_t n,
@@ -1139,15 +1188,15 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, bool init, u
                        stream, b.keys_in, n, G, b.row_count, b.row_fill, b.row_start, b.pairs);
     if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(row_sort_kernel, dim3(G), dim3(256), 0, stream, b.pairs, b.scratch_pairs,
-                       b.row_start, b.row_count, b.row_fill, G, n, b.perm, b.off);
+                       b.row_start, b.row_count, b.row_fill, G, n, b.perm, b.off, b.pos_i,
+                       b.vfme_i, b.spos, b.svfme);
     if ((e = mark(2)) != hipSuccess) return e;
     if (c.enable_spring) {
-        hipLaunchKernelGGL(physics_step_kernel<true>, grid, dim3(256), 0, stream, c, b.perm, b.off,
-                           b.keys_out, b.row_count, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, n);
+        hipLaunchKernelGGL(physics_step_kernel<true>, grid, dim3(256), 0, stream, c, b.off,
+                           b.keys_out, b.row_count, b.spos, b.svfme, b.pos_o, b.vfme_o, n);
     } else {
-        hipLaunchKernelGGL(physics_step_kernel<false>, grid, dim3(256), 0, stream, c, b.perm,
-                           b.off, b.keys_out, b.row_count, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o,
-                           n);
+        hipLaunchKernelGGL(physics_step_kernel<false>, grid, dim3(256), 0, stream, c, b.off,
+                           b.keys_out, b.row_count, b.spos, b.svfme, b.pos_o, b.vfme_o, n);
     }
     if ((e = mark(3)) != hipSuccess) return e;
     return hipGetLastError();
