@@ -1006,11 +1006,6 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
     }
 }
 
-struct StageWindows {
-    uint32_t lo[3];
-    uint32_t cnt[3];
-};
-
 template <bool SPRING>
 __device__ __forceinline__ void collide_candidate(const PhysConsts &c, Agent &a, float4 bp,
                                                   const uint4 *__restrict__ svfme, uint32_t i)
@@ -1027,6 +1022,45 @@ __device__ __forceinline__ void collide_candidate(const PhysConsts &c, Agent &a,
     if (d2 <= (edge * edge) * 1.0001f) solve_sphere<SPRING>(c, a, bp, svfme, i);
 }
 
+struct StageWindows {
+    uint32_t lo[3];
+    uint32_t cnt[3];
+};
+
+// Walks the candidates of one sphere: middle, bottom and top ranges in the
+// shader's order (physics.wgsl:141-147, 200-205), skipping its own slot.
+// FROM_LDS reads the neighbour positions from the staged windows (every
+// range lies inside its window), otherwise from the sorted state in global
+// memory.  The two variants are separate code paths so that the compiler
+// never merges the LDS and global reads into one flat access.
+template <bool SPRING, bool FROM_LDS>
+__device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, uint32_t s,
+                                                const uint32_t (&lo)[3], const uint32_t (&cnt)[3],
+                                                const uint32_t (&wlo)[3],
+                                                const float4 *__restrict__ s_pos,
+                                                const float4 *__restrict__ spos,
+                                                const uint4 *__restrict__ svfme)
+{
+    const uint32_t n0 = cnt[0];
+    const uint32_t n01 = n0 + cnt[1];
+    const uint32_t total = n01 + cnt[2];
+    for (uint32_t j = 0; j < total; ++j) {
+        const bool b1 = j >= n0, b2 = j >= n01;
+        const uint32_t l = b2 ? lo[2] : (b1 ? lo[1] : lo[0]);
+        const uint32_t st = b2 ? n01 : (b1 ? n0 : 0u);
+        const uint32_t i = l + (j - st);
+        float4 bp;
+        if (FROM_LDS) {
+            const uint32_t wl = b2 ? wlo[2] : (b1 ? wlo[1] : wlo[0]);
+            const uint32_t wb = b2 ? 2u * kStageCap : (b1 ? kStageCap : 0u);
+            bp = s_pos[wb + (i - wl)];
+        } else {
+            bp = spos[i];
+        }
+        if (i != s) collide_candidate<SPRING>(c, a, bp, svfme, i);
+    }
+}
+
 // main() with LOOP_PIPELINE for a block of 256 consecutive sorted slots, on
 // the state already gathered into sorted order.  The slots' cells are
 // non-decreasing, so the candidates of the whole block in each of the three
@@ -1040,7 +1074,7 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     const uint4 *__restrict__ svfme, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
     uint32_t n)
 {
-    __shared__ float4 s_pos[3][kStageCap];
+    __shared__ float4 s_pos[3 * kStageCap];  // the three windows back to back
     __shared__ StageWindows s_win;
     __shared__ uint32_t s_kfirst, s_klast;
 
@@ -1075,6 +1109,8 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     }
     __syncthreads();
     if (tid < 3u) {
+        // window w: every slot of cells [kfirst + dk - 1, klast + dk + 1],
+        // a superset of each thread's range in that row
         const int64_t dk = tid == 0 ? 0 : (tid == 1 ? -(int64_t)G : (int64_t)G);
         int64_t lc = (int64_t)s_kfirst + dk - 1;
         int64_t hc = (int64_t)s_klast + dk + 2;
@@ -1083,38 +1119,30 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
         const uint32_t lo = off[lc];
         const uint32_t hi = max(off[hc], lo);
         s_win.lo[tid] = lo;
-        s_win.cnt[tid] = min(hi - lo, kStageCap);
+        s_win.cnt[tid] = hi - lo;
     }
     __syncthreads();
+    const bool staged = s_win.cnt[0] <= kStageCap && s_win.cnt[1] <= kStageCap &&
+                        s_win.cnt[2] <= kStageCap;
+    if (staged) {
 #pragma unroll
-    for (uint32_t w = 0; w < 3; ++w) {
-        const uint32_t lo = s_win.lo[w], cnt = s_win.cnt[w];
-        for (uint32_t j = tid; j < cnt; j += 256u) s_pos[w][j] = spos[lo + j];
+        for (uint32_t w = 0; w < 3; ++w) {
+            const uint32_t lo = s_win.lo[w], cnt = s_win.cnt[w];
+            for (uint32_t j = tid; j < cnt; j += 256u) s_pos[w * kStageCap + j] = spos[lo + j];
+        }
     }
     __syncthreads();
     if (!valid) return;
 
-    const uint32_t n0 = hi0 - lo0, n1 = hi1 - lo1, n2 = hi2 - lo2;
-    const uint32_t total = n0 + n1 + n2;
+    const uint32_t lo[3] = {lo0, lo1, lo2};
+    const uint32_t cnt[3] = {hi0 - lo0, hi1 - lo1, hi2 - lo2};
+    const uint32_t wlo[3] = {s_win.lo[0], s_win.lo[1], s_win.lo[2]};
     verlet(c, a);
     const float ox = a.px, oy = a.py, oz = a.pz;
-    for (uint32_t j = 0; j < total; ++j) {
-        uint32_t i, w;
-        if (j < n0) {
-            i = lo0 + j;
-            w = 0;
-        } else if (j < n0 + n1) {
-            i = lo1 + (j - n0);
-            w = 1;
-        } else {
-            i = lo2 + (j - n0 - n1);
-            w = 2;
-        }
-        if (i == s) continue;  // only possible in the middle row
-        const uint32_t rel = i - s_win.lo[w];
-        const float4 bp = rel < s_win.cnt[w] ? s_pos[w][rel] : spos[i];
-        collide_candidate<SPRING>(c, a, bp, svfme, i);
-    }
+    if (staged)
+        walk_candidates<SPRING, true>(c, a, s, lo, cnt, wlo, s_pos, spos, svfme);
+    else
+        walk_candidates<SPRING, false>(c, a, s, lo, cnt, wlo, s_pos, spos, svfme);
     jitter(a, s, ox, oy, oz);
     const uint32_t key = finish_agent<SPRING>(c, a, s, pos_o, vfme_o, key_out);
     wave_count_global(row_count, key / G, true);
