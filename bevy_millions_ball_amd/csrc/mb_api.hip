@@ -302,9 +302,7 @@ int mb_physics_set_settings(mb_physics *p, const mb_settings *s)
         // the stale keys of the old grid once, reading stale `close` words).
         DeviceGuard g(p->device);
         const int cur = (int)((p->time + 1) & 1);
-        MB_HIP(hipMemset(p->rows, 0, 2 * (size_t)mb::kMaxRows * 4u));
-        hipError_t e = mb::launch_rekey(p->consts, p->pos[cur], p->keys[cur], p->row_count(),
-                                        p->n, nullptr);
+        hipError_t e = mb::launch_rekey(p->consts, p->pos[cur], p->keys[cur], p->n, nullptr);
         if (e != hipSuccess) return fail_hip(e, "rekey");
         MB_HIP(hipStreamSynchronize(nullptr));
     }

@@ -63,7 +63,7 @@ hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
 
 // ---- frame path ------------------------------------------------------------
 constexpr uint32_t kMaxRows = 4096;     // G <= 4096 (G^2 <= 2^24)
-constexpr uint32_t kPartItems = 16;
+constexpr uint32_t kPartItems = 4;
 constexpr uint32_t kPartTile = 256 * kPartItems;  // keys per partition block
 constexpr uint32_t kRowCap = 4096;      // spheres of one row sorted in LDS
 constexpr uint32_t kStageCap = 512;     // slots per staged neighbour-row window
@@ -73,7 +73,7 @@ struct FrameBuffers {
     uint32_t *keys_out;        // cell keys of the output state
     uint32_t *perm;            // sorted slot -> index into the input state
     uint32_t *off;             // G^2 + 1 per-cell slot offsets
-    uint32_t *row_count;       // kMaxRows: spheres per row of keys_in
+    uint32_t *row_count;       // kMaxRows: spheres per row of keys_in (zero on entry)
     uint32_t *row_fill;        // kMaxRows: partition cursors
     uint32_t *row_start;       // kMaxRows + 1: exclusive scan of row_count
     uint64_t *pairs;           // n: (cx, index) partitioned by row
@@ -86,17 +86,17 @@ struct FrameBuffers {
     uint4 *vfme_o;
 };
 
-// Enqueues one frame: INIT (offsets fill + copy kernel) or LOOP (row
-// partition, row sort, collide + integrate).  row_count must hold the row
-// histogram of keys_in for a LOOP frame and be zero before an INIT frame.
-// `events` (4 entries or null) are recorded before the partition, before the
+// Enqueues one frame: INIT (offsets fill + copy kernel) or LOOP (row count,
+// row partition, row sort, collide + integrate).  row_count and row_fill must
+// be zero on entry; the row sort leaves them zero.
+// `events` (4 entries or null) are recorded before the row count, before the
 // row sort, before collide + integrate and after it.
 hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, bool init, uint32_t n,
                         hipStream_t stream, hipEvent_t *events);
 
-// key_out[i] = cell key of pos[i]; adds the keys' rows to row_count.
-hipError_t launch_rekey(const PhysConsts &c, const float4 *pos, uint32_t *key_out,
-                        uint32_t *row_count, uint32_t n, hipStream_t stream);
+// key_out[i] = cell key of pos[i].
+hipError_t launch_rekey(const PhysConsts &c, const float4 *pos, uint32_t *key_out, uint32_t n,
+                        hipStream_t stream);
 
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t count, hipStream_t stream);
 

@@ -760,6 +760,7 @@ hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
 // A cell key is cz * G + cx, so the stable sort of the keys is the stable
 // partition of the spheres by row cz followed, inside each row, by the order
 // (cx, input index).  The frame path exploits that:
+//   row_count_kernel      row histogram of the input keys;
 //   row_partition_kernel  scatters (cx, index) pairs into their row's slot
 //                         range; each block reserves one sub-range per row it
 //                         touches (order inside a row is arbitrary);
@@ -770,8 +771,9 @@ hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
 //                         state gathered into sorted order;
 //   physics_step_kernel   collide + integrate on the sorted state with the
 //                         neighbour-row windows staged in LDS; it writes the
-//                         next keys and adds them to the next frame's row
-//                         histogram.
+//                         next keys.
+// Global atomics are limited to one per (block, row touched): many waves
+// adding to one counter serialise badly on this part.
 // No stage needs a cross-block ordering: there is no look-back and no spin.
 
 // Physical block b runs on XCD b % 8 (round-robin dispatch); give each XCD a
@@ -807,21 +809,6 @@ __device__ __forceinline__ uint32_t wave_slot_lds(uint32_t *counter, uint32_t ke
     return slot;
 }
 
-// Global counter[key] += 1 for every active lane, one non-returning atomic
-// per distinct key of the wave.
-__device__ __forceinline__ void wave_count_global(uint32_t *counter, uint32_t key, bool active)
-{
-    const uint32_t lane = threadIdx.x & 63u;
-    uint64_t todo = __ballot(active);
-    while (todo) {
-        const uint32_t leader = (uint32_t)__ffsll((long long)todo) - 1u;
-        const uint32_t lk = (uint32_t)__shfl((int)key, (int)leader, 64);
-        const uint64_t peers = __ballot(active && key == lk);
-        if (lane == leader) atomicAdd(&counter[lk], (uint32_t)__popcll(peers));
-        todo &= ~peers;
-    }
-}
-
 // Exclusive scan, in place, of `count` (<= 4096) LDS words with a 256-thread
 // block.  Returns the total.  `scratch` is 4 LDS words.
 __device__ __forceinline__ uint32_t block256_scan_lds(uint32_t *vals, uint32_t count,
@@ -846,6 +833,65 @@ __device__ __forceinline__ uint32_t block256_scan_lds(uint32_t *vals, uint32_t c
     return total;
 }
 
+// Smallest and largest row among the block's keys (block-wide, via LDS).
+__device__ __forceinline__ void block_row_span(uint32_t lo, uint32_t hi, uint32_t *s_lo,
+                                               uint32_t *s_hi)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (uint32_t off = 32; off > 0; off >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, (int)off, 64));
+        hi = max(hi, (uint32_t)__shfl_xor((int)hi, (int)off, 64));
+    }
+    if (threadIdx.x == 0) {
+        *s_lo = 0xFFFFFFFFu;
+        *s_hi = 0u;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        atomicMin(s_lo, lo);
+        atomicMax(s_hi, hi);
+    }
+    __syncthreads();
+}
+
+// Row histogram of the frame's input keys: each block counts its tile in LDS
+// and adds one atomic per row it touches (nearly sorted keys touch few rows,
+// so every row counter sees about as many atomics as blocks overlap it).
+__global__ __launch_bounds__(256) void row_count_kernel(const uint32_t *__restrict__ keys,
+                                                        uint32_t n, uint32_t grids,
+                                                        uint32_t *__restrict__ row_count)
+{
+    __shared__ uint32_t s_hist[kMaxRows];
+    __shared__ uint32_t s_lo, s_hi;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t base = blockIdx.x * kPartTile;
+    uint32_t row[kPartItems];
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < kPartItems; ++j) {
+        const uint32_t i = base + j * 256u + tid;
+        row[j] = i < n ? min(keys[i] / grids, grids - 1u) : kInvalid;
+        if (i < n) {
+            lo = min(lo, row[j]);
+            hi = max(hi, row[j]);
+        }
+    }
+    block_row_span(lo, hi, &s_lo, &s_hi);
+    const uint32_t rlo = s_lo, rhi = s_hi;
+    if (rlo > rhi) return;  // empty tile (uniform across the block)
+    for (uint32_t r = rlo + tid; r <= rhi; r += 256u) s_hist[r] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < kPartItems; ++j)
+        (void)wave_slot_lds(s_hist, row[j], row[j] != kInvalid);
+    __syncthreads();
+    for (uint32_t r = rlo + tid; r <= rhi; r += 256u) {
+        const uint32_t c = s_hist[r];
+        if (c) atomicAdd(&row_count[r], c);
+    }
+}
+
 __global__ __launch_bounds__(256) void row_partition_kernel(
     const uint32_t *__restrict__ keys, uint32_t n, uint32_t grids,
     const uint32_t *__restrict__ row_count, uint32_t *__restrict__ row_fill,
@@ -854,12 +900,10 @@ __global__ __launch_bounds__(256) void row_partition_kernel(
     __shared__ uint32_t s_start[kMaxRows + 1];
     __shared__ uint32_t s_mine[kMaxRows];  // this block's count, then its base, per row
     __shared__ uint32_t s_scratch[4];
+    __shared__ uint32_t s_lo, s_hi;
     const uint32_t tid = threadIdx.x;
     const uint32_t rows = grids;
-    for (uint32_t r = tid; r < rows; r += 256u) {
-        s_start[r] = row_count[r];
-        s_mine[r] = 0u;
-    }
+    for (uint32_t r = tid; r < rows; r += 256u) s_start[r] = row_count[r];
     __syncthreads();
     const uint32_t total = block256_scan_lds(s_start, rows, s_scratch);
     if (tid == 0) s_start[rows] = total;
@@ -870,11 +914,22 @@ __global__ __launch_bounds__(256) void row_partition_kernel(
     const uint32_t base = blockIdx.x * kPartTile;
     uint32_t key[kPartItems];
     uint32_t slot[kPartItems];
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
 #pragma unroll
     for (uint32_t j = 0; j < kPartItems; ++j) {
         const uint32_t i = base + j * 256u + tid;
         key[j] = i < n ? keys[i] : 0u;
+        if (i < n) {
+            const uint32_t r = min(key[j] / grids, rows - 1u);
+            lo = min(lo, r);
+            hi = max(hi, r);
+        }
     }
+    block_row_span(lo, hi, &s_lo, &s_hi);
+    const uint32_t rlo = s_lo, rhi = s_hi;
+    if (rlo > rhi) return;
+    for (uint32_t r = rlo + tid; r <= rhi; r += 256u) s_mine[r] = 0u;
+    __syncthreads();
 #pragma unroll
     for (uint32_t j = 0; j < kPartItems; ++j) {
         const uint32_t i = base + j * 256u + tid;
@@ -883,7 +938,7 @@ __global__ __launch_bounds__(256) void row_partition_kernel(
     }
     __syncthreads();
     // One reservation per row this block touches.
-    for (uint32_t r = tid; r < rows; r += 256u) {
+    for (uint32_t r = rlo + tid; r <= rhi; r += 256u) {
         const uint32_t c = s_mine[r];
         if (c) s_mine[r] = s_start[r] + atomicAdd(&row_fill[r], c);
     }
@@ -1070,7 +1125,7 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
 template <bool SPRING>
 __global__ __launch_bounds__(256) void physics_step_kernel(
     PhysConsts c, const uint32_t *__restrict__ off, uint32_t *__restrict__ key_out,
-    uint32_t *__restrict__ row_count, const float4 *__restrict__ spos,
+    const float4 *__restrict__ spos,
     const uint4 *__restrict__ svfme, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
     uint32_t n)
 {
@@ -1144,15 +1199,14 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     else
         walk_candidates<SPRING, false>(c, a, s, lo, cnt, wlo, s_pos, spos, svfme);
     jitter(a, s, ox, oy, oz);
-    const uint32_t key = finish_agent<SPRING>(c, a, s, pos_o, vfme_o, key_out);
-    wave_count_global(row_count, key / G, true);
+    (void)finish_agent<SPRING>(c, a, s, pos_o, vfme_o, key_out);
 }
 
 // INIT frame of the frame path: the sort of the all-zero keys is the
 // identity, so perm[s] = s and every sphere sits in cell 0's range.
 __global__ __launch_bounds__(256) void physics_init_step_kernel(
     PhysConsts c, uint32_t *__restrict__ perm, uint32_t *__restrict__ key_out,
-    uint32_t *__restrict__ row_count, const float4 *__restrict__ pos_i,
+    const float4 *__restrict__ pos_i,
     const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
     uint32_t n)
 {
@@ -1167,7 +1221,6 @@ __global__ __launch_bounds__(256) void physics_init_step_kernel(
     vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
                            pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
     key_out[s] = cz * c.grids + cx;
-    wave_count_global(row_count, cz, true);
 }
 
 // Offsets of the INIT frame: cell 0 holds every sphere.
@@ -1179,8 +1232,7 @@ __global__ __launch_bounds__(256) void init_offsets_kernel(uint32_t *__restrict_
 }
 
 __global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *__restrict__ pos,
-                                                    uint32_t *__restrict__ key_out,
-                                                    uint32_t *__restrict__ row_count, uint32_t n)
+                                                    uint32_t *__restrict__ key_out, uint32_t n)
 {
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;
     if (s >= n) return;
@@ -1188,7 +1240,6 @@ __global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *
     uint32_t cx, cz;
     cell_of(c, p.x, p.z, cx, cz);
     key_out[s] = cz * c.grids + cx;
-    wave_count_global(row_count, cz, true);
 }
 
 hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, bool init, uint32_t n,
@@ -1198,6 +1249,7 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, bool init, u
     const uint32_t G = c.grids;
     const uint32_t cells = G * G;
     const dim3 grid((n + 255u) / 256u);
+    const dim3 tiles((n + kPartTile - 1u) / kPartTile);
     auto mark = [&](int i) {
         return events ? hipEventRecord(events[i], stream) : hipSuccess;
     };
@@ -1208,12 +1260,14 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, bool init, u
                            b.off, cells, n);
         if ((e = mark(1)) != hipSuccess || (e = mark(2)) != hipSuccess) return e;
         hipLaunchKernelGGL(physics_init_step_kernel, grid, dim3(256), 0, stream, c, b.perm,
-                           b.keys_out, b.row_count, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, n);
+                           b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, n);
         if ((e = mark(3)) != hipSuccess) return e;
         return hipGetLastError();
     }
-    hipLaunchKernelGGL(row_partition_kernel, dim3((n + kPartTile - 1u) / kPartTile), dim3(256), 0,
-                       stream, b.keys_in, n, G, b.row_count, b.row_fill, b.row_start, b.pairs);
+    hipLaunchKernelGGL(row_count_kernel, tiles, dim3(256), 0, stream, b.keys_in, n, G,
+                       b.row_count);
+    hipLaunchKernelGGL(row_partition_kernel, tiles, dim3(256), 0, stream, b.keys_in, n, G,
+                       b.row_count, b.row_fill, b.row_start, b.pairs);
     if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(row_sort_kernel, dim3(G), dim3(256), 0, stream, b.pairs, b.scratch_pairs,
                        b.row_start, b.row_count, b.row_fill, G, n, b.perm, b.off, b.pos_i,
@@ -1221,21 +1275,21 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, bool init, u
     if ((e = mark(2)) != hipSuccess) return e;
     if (c.enable_spring) {
         hipLaunchKernelGGL(physics_step_kernel<true>, grid, dim3(256), 0, stream, c, b.off,
-                           b.keys_out, b.row_count, b.spos, b.svfme, b.pos_o, b.vfme_o, n);
+                           b.keys_out, b.spos, b.svfme, b.pos_o, b.vfme_o, n);
     } else {
         hipLaunchKernelGGL(physics_step_kernel<false>, grid, dim3(256), 0, stream, c, b.off,
-                           b.keys_out, b.row_count, b.spos, b.svfme, b.pos_o, b.vfme_o, n);
+                           b.keys_out, b.spos, b.svfme, b.pos_o, b.vfme_o, n);
     }
     if ((e = mark(3)) != hipSuccess) return e;
     return hipGetLastError();
 }
 
-hipError_t launch_rekey(const PhysConsts &c, const float4 *pos, uint32_t *key_out,
-                        uint32_t *row_count, uint32_t n, hipStream_t stream)
+hipError_t launch_rekey(const PhysConsts &c, const float4 *pos, uint32_t *key_out, uint32_t n,
+                        hipStream_t stream)
 {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(rekey_kernel, dim3((n + 255u) / 256u), dim3(256), 0, stream, c, pos,
-                       key_out, row_count, n);
+                       key_out, n);
     return hipGetLastError();
 }
 
