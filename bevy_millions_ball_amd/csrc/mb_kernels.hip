@@ -1084,35 +1084,98 @@ struct StageWindows {
 
 // Walks the candidates of one sphere: middle, bottom and top ranges in the
 // shader's order (physics.wgsl:141-147, 200-205), skipping its own slot.
+//
+// The sphere's state changes only at a contact, and a contact needs
+// |A - B| <= r_A + r_B.  Each chunk of up to 64 candidates is therefore first
+// scanned against the position `ref` the sphere had at the start of the
+// chunk, with the reach widened by a margin M = r_A / 4: a candidate outside
+// that reach cannot be in contact while |A - ref| <= M.  Only the candidates
+// inside it (a bitmask) go through solve_sphere_constraint, in order; if a
+// contact moves the sphere by more than M the scan restarts right after that
+// candidate from the new position.  The sequence of state changes is the
+// shader's; only the wasted work on distant candidates is gone, and with it
+// most of the wave divergence of the contact path.
+//
 // FROM_LDS reads the neighbour positions from the staged windows (every
 // range lies inside its window), otherwise from the sorted state in global
 // memory.  The two variants are separate code paths so that the compiler
 // never merges the LDS and global reads into one flat access.
+// A sphere's candidates, numbered j = 0 .. total-1 over its middle, bottom
+// and top ranges in that order: candidate j is sorted slot
+// j + slot0 + (j >= n0 ? slot1 : 0) + (j >= n01 ? slot2 : 0), and its staged
+// copy sits at LDS index j + lds0 + (j >= n0 ? lds1 : 0) + (j >= n01 ? lds2 : 0)
+// (u32 arithmetic, wrapping).  Plain adds of selected scalars keep the
+// compiler from turning the three-way choice into an indexed private array.
+struct Ranges {
+    uint32_t n0, n01, total;
+    uint32_t slot0, slot1, slot2;
+    uint32_t lds0, lds1, lds2;
+};
+
+__device__ __forceinline__ Ranges make_ranges(uint32_t lo0, uint32_t hi0, uint32_t lo1,
+                                              uint32_t hi1, uint32_t lo2, uint32_t hi2,
+                                              uint32_t wlo0, uint32_t wlo1, uint32_t wlo2)
+{
+    Ranges r;
+    r.n0 = hi0 - lo0;
+    r.n01 = r.n0 + (hi1 - lo1);
+    r.total = r.n01 + (hi2 - lo2);
+    // slot of candidate j in range w: lo_w + (j - start_w)
+    const uint32_t b0 = lo0, b1 = lo1 - r.n0, b2 = lo2 - r.n01;
+    r.slot0 = b0;
+    r.slot1 = b1 - b0;
+    r.slot2 = b2 - b1;
+    // LDS index: w * kStageCap + (slot - wlo_w)
+    const uint32_t l0 = b0 - wlo0, l1 = kStageCap + b1 - wlo1, l2 = 2u * kStageCap + b2 - wlo2;
+    r.lds0 = l0;
+    r.lds1 = l1 - l0;
+    r.lds2 = l2 - l1;
+    return r;
+}
+
 template <bool SPRING, bool FROM_LDS>
 __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, uint32_t s,
-                                                const uint32_t (&lo)[3], const uint32_t (&cnt)[3],
-                                                const uint32_t (&wlo)[3],
+                                                const Ranges r,
                                                 const float4 *__restrict__ s_pos,
                                                 const float4 *__restrict__ spos,
                                                 const uint4 *__restrict__ svfme)
 {
-    const uint32_t n0 = cnt[0];
-    const uint32_t n01 = n0 + cnt[1];
-    const uint32_t total = n01 + cnt[2];
-    for (uint32_t j = 0; j < total; ++j) {
-        const bool b1 = j >= n0, b2 = j >= n01;
-        const uint32_t l = b2 ? lo[2] : (b1 ? lo[1] : lo[0]);
-        const uint32_t st = b2 ? n01 : (b1 ? n0 : 0u);
-        const uint32_t i = l + (j - st);
-        float4 bp;
-        if (FROM_LDS) {
-            const uint32_t wl = b2 ? wlo[2] : (b1 ? wlo[1] : wlo[0]);
-            const uint32_t wb = b2 ? 2u * kStageCap : (b1 ? kStageCap : 0u);
-            bp = s_pos[wb + (i - wl)];
-        } else {
-            bp = spos[i];
+    const uint32_t total = r.total;
+    auto candidate = [&](uint32_t j, uint32_t &i) -> float4 {
+        const bool b1 = j >= r.n0, b2 = j >= r.n01;
+        i = j + r.slot0 + (b1 ? r.slot1 : 0u) + (b2 ? r.slot2 : 0u);
+        if (FROM_LDS) return s_pos[j + r.lds0 + (b1 ? r.lds1 : 0u) + (b2 ? r.lds2 : 0u)];
+        return spos[i];
+    };
+    const float margin = 0.25f * a.scale;
+    const float still = (margin * 0.999f) * (margin * 0.999f);
+    uint32_t j = 0;
+    while (j < total) {
+        const uint32_t end = min(j + 64u, total);
+        const float rx = a.px, ry = a.py, rz = a.pz;
+        uint64_t near = 0;
+        for (uint32_t t = j; t < end; ++t) {
+            uint32_t i;
+            const float4 bp = candidate(t, i);
+            const float reach = (bp.w + a.scale) + margin;
+            const float dx = rx - bp.x, dy = ry - bp.y, dz = rz - bp.z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if (i != s && d2 <= (reach * reach) * 1.0001f) near |= 1ull << (t - j);
         }
-        if (i != s) collide_candidate<SPRING>(c, a, bp, svfme, i);
+        uint32_t next = end;
+        while (near) {
+            const uint32_t t = j + (uint32_t)__builtin_ctzll(near);
+            near &= near - 1ull;
+            uint32_t i;
+            const float4 bp = candidate(t, i);
+            collide_candidate<SPRING>(c, a, bp, svfme, i);
+            const float ex = a.px - rx, ey = a.py - ry, ez = a.pz - rz;
+            if (!(ex * ex + ey * ey + ez * ez <= still)) {
+                next = t + 1u;
+                break;
+            }
+        }
+        j = next;
     }
 }
 
@@ -1189,15 +1252,14 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     __syncthreads();
     if (!valid) return;
 
-    const uint32_t lo[3] = {lo0, lo1, lo2};
-    const uint32_t cnt[3] = {hi0 - lo0, hi1 - lo1, hi2 - lo2};
-    const uint32_t wlo[3] = {s_win.lo[0], s_win.lo[1], s_win.lo[2]};
+    const Ranges rg =
+        make_ranges(lo0, hi0, lo1, hi1, lo2, hi2, s_win.lo[0], s_win.lo[1], s_win.lo[2]);
     verlet(c, a);
     const float ox = a.px, oy = a.py, oz = a.pz;
     if (staged)
-        walk_candidates<SPRING, true>(c, a, s, lo, cnt, wlo, s_pos, spos, svfme);
+        walk_candidates<SPRING, true>(c, a, s, rg, s_pos, spos, svfme);
     else
-        walk_candidates<SPRING, false>(c, a, s, lo, cnt, wlo, s_pos, spos, svfme);
+        walk_candidates<SPRING, false>(c, a, s, rg, s_pos, spos, svfme);
     jitter(a, s, ox, oy, oz);
     (void)finish_agent<SPRING>(c, a, s, pos_o, vfme_o, key_out);
 }
