@@ -94,6 +94,8 @@ mb::PhysConsts compile_consts(const mb_settings &s)
     c.neg_c_over_dt = -c.c_agent / c.t_delta;
     c.neg_1pe_over_dt = -(1.0f + c.e_envir) / c.t_delta;
     c.enable_spring = s.enable_spring ? 1 : 0;
+    c.grid_size_div = mb::make_fastdiv(s.grid_size);
+    c.grids_div = mb::make_fastdiv(s.number_of_grids_one_dimension);
     return c;
 }
 

@@ -9,6 +9,40 @@ namespace mb {
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 
+// Division of any u32 by a fixed divisor d >= 1 with one mulhi (Granlund and
+// Montgomery's method for divisors that need a 33-bit multiplier): with
+// l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1,
+// n / d = (t + ((n - t) >> 1)) >> (l - 1) where t = mulhi(n, m).
+// d = 1 uses m = 0 and shifts 0 / 0.
+struct FastDiv {
+    uint32_t mul;
+    uint32_t sh1;
+    uint32_t sh2;
+};
+
+inline FastDiv make_fastdiv(uint32_t d)
+{
+    FastDiv f;
+    if (d <= 1u) {
+        f.mul = 0u;
+        f.sh1 = 0u;
+        f.sh2 = 0u;
+        return f;
+    }
+    uint32_t l = 0;
+    while (l < 32u && (1ull << l) < d) ++l;
+    f.mul = (uint32_t)((((1ull << 32) * ((1ull << l) - d)) / d) + 1ull);
+    f.sh1 = 1u;
+    f.sh2 = l - 1u;
+    return f;
+}
+
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, const FastDiv &f)
+{
+    const uint32_t t = __umulhi(n, f.mul);
+    return (t + ((n - t) >> f.sh1)) >> f.sh2;
+}
+
 // The constants physics.wgsl:2-19 sees after CompiledConsts::shader_defs
 // (src/physics/mod.rs:459-485), plus two products that the shader forms on
 // every use and that are computed once here with the same f32 operations.
@@ -27,6 +61,8 @@ struct PhysConsts {
     float neg_c_over_dt;     // -C_AGENT / T_DELTA   (physics.wgsl:285, 324)
     float neg_1pe_over_dt;   // -(1. + E_ENVIR) / T_DELTA (physics.wgsl:292)
     int32_t enable_spring;
+    FastDiv grid_size_div;   // u32 / GRID_SIZE
+    FastDiv grids_div;       // cell key / NUMBER_OF_GRIDS_ONE_DIMENSION = row
 };
 
 // ---- radix sort (8-bit digits, reduce-then-scan per pass) ----------------

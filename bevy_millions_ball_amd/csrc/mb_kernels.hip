@@ -408,6 +408,17 @@ __device__ __forceinline__ float sign_sel(float x)
     return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f);
 }
 
+// Divisor d as its exact reciprocal when d is a power of two whose reciprocal
+// is a normal float; x * (1/d) then rounds the same real number as x / d, so
+// the result is bit-identical to the IEEE division.  Returns 0 otherwise.
+__device__ __forceinline__ float exact_reciprocal(float d)
+{
+    const uint32_t b = __float_as_uint(d);
+    const uint32_t e = (b >> 23) & 0xFFu;
+    if ((b & 0x7FFFFFu) != 0u || e < 1u || e > 253u) return 0.0f;
+    return __uint_as_float((b & 0x80000000u) | ((254u - e) << 23));
+}
+
 // WGSL u32(f32): truncation toward zero, saturating, NaN -> 0.
 __device__ __forceinline__ uint32_t f32_to_u32_sat(float x)
 {
@@ -438,8 +449,8 @@ __device__ __forceinline__ uint32_t pack_half2(float lo, float hi)
 __device__ __forceinline__ void cell_of(const PhysConsts &c, float x, float z, uint32_t &cx,
                                         uint32_t &cz)
 {
-    const uint32_t ix = f32_to_u32_sat(x + c.half_map_size) / c.grid_size;
-    const uint32_t iz = f32_to_u32_sat(z + c.half_map_size) / c.grid_size;
+    const uint32_t ix = fast_div(f32_to_u32_sat(x + c.half_map_size), c.grid_size_div);
+    const uint32_t iz = fast_div(f32_to_u32_sat(z + c.half_map_size), c.grid_size_div);
     cx = min(ix, c.grids - 1u);
     cz = min(iz, c.grids - 1u);
 }
@@ -467,24 +478,22 @@ __device__ __forceinline__ Agent agent_from(float4 p, uint4 q)
     return a;
 }
 
-// solve_sphere_constraint (physics.wgsl:300-331) against the neighbour in
-// sorted slot i.  The neighbour's packed state is read only when it is needed
-// (contact, rigid response).
+// solve_sphere_constraint (physics.wgsl:300-331) from the offset
+// t = A.position - B.position, d2 = dot(t, t) and edge = r_B + r_A, against
+// the neighbour whose packed state is vfme[bo].  The neighbour's packed state
+// is read only when it is needed (contact, rigid response).
 template <bool SPRING>
-__device__ __forceinline__ void solve_sphere(const PhysConsts &c, Agent &a, float4 bp,
-                                             const uint4 *__restrict__ vfme_i, uint32_t bo)
+__device__ __forceinline__ void sphere_contact(const PhysConsts &c, Agent &a, float edge,
+                                               float tx, float ty, float tz, float d2,
+                                               const uint4 *__restrict__ vfme, uint32_t bo)
 {
-    const float edge = bp.w + a.scale;
-    const float tx = a.px - bp.x;
-    const float ty = a.py - bp.y;
-    const float tz = a.pz - bp.z;
-    const float dist = sqrtf(tx * tx + ty * ty + tz * tz);
+    const float dist = sqrtf(d2);
     if (dist <= edge) {
         const float nx = tx / dist;
         const float ny = ty / dist;
         const float nz = tz / dist;
         if (!SPRING) {
-            const uint4 q = vfme_i[bo];
+            const uint4 q = vfme[bo];
             const float bvx = half_lo(q.x), bvy = half_hi(q.x), bvz = half_lo(q.y);
             const float m1 = a.m;
             const float m2 = half_lo(q.w);
@@ -496,9 +505,17 @@ __device__ __forceinline__ void solve_sphere(const PhysConsts &c, Agent &a, floa
                 const float rnx = rspd * nx, rny = rspd * ny, rnz = rspd * nz;
                 const float coef = m1 - c.e_agent * m2;
                 const float msum = m1 + m2;
-                const float rcx = (coef * rnx) / msum;
-                const float rcy = (coef * rny) / msum;
-                const float rcz = (coef * rnz) / msum;
+                const float inv = exact_reciprocal(msum);
+                float rcx, rcy, rcz;
+                if (inv != 0.0f) {
+                    rcx = (coef * rnx) * inv;
+                    rcy = (coef * rny) * inv;
+                    rcz = (coef * rnz) * inv;
+                } else {
+                    rcx = (coef * rnx) / msum;
+                    rcy = (coef * rny) / msum;
+                    rcz = (coef * rnz) / msum;
+                }
                 a.vx = (bvx + (rx - rnx)) + rcx;
                 a.vy = (bvy + (ry - rny)) + rcy;
                 a.vz = (bvz + (rz - rnz)) + rcz;
@@ -521,6 +538,17 @@ __device__ __forceinline__ void solve_sphere(const PhysConsts &c, Agent &a, floa
             }
         }
     }
+}
+
+template <bool SPRING>
+__device__ __forceinline__ void solve_sphere(const PhysConsts &c, Agent &a, float4 bp,
+                                             const uint4 *__restrict__ vfme, uint32_t bo)
+{
+    const float edge = bp.w + a.scale;
+    const float tx = a.px - bp.x;
+    const float ty = a.py - bp.y;
+    const float tz = a.pz - bp.z;
+    sphere_contact<SPRING>(c, a, edge, tx, ty, tz, tx * tx + ty * ty + tz * tz, vfme, bo);
 }
 
 template <bool SPRING>
@@ -621,7 +649,17 @@ __device__ __forceinline__ void solve_cube(const PhysConsts &c, Agent &a)
 // verlet_integration (physics.wgsl:262-266) followed by `force = 0` (:78).
 __device__ __forceinline__ void verlet(const PhysConsts &c, Agent &a)
 {
-    const float ax = a.fx / a.m, ay = a.fy / a.m, az = a.fz / a.m;
+    float ax, ay, az;
+    const float inv = exact_reciprocal(a.m);
+    if (inv != 0.0f) {
+        ax = a.fx * inv;
+        ay = a.fy * inv;
+        az = a.fz * inv;
+    } else {
+        ax = a.fx / a.m;
+        ay = a.fy / a.m;
+        az = a.fz / a.m;
+    }
     a.vx = a.vx + ((c.gravity_x + ax) * c.t_delta) * c.t_delta;
     a.vy = a.vy + ((c.gravity_y + ay) * c.t_delta) * c.t_delta;
     a.vz = a.vz + ((c.gravity_z + az) * c.t_delta) * c.t_delta;
@@ -659,12 +697,15 @@ __device__ __forceinline__ uint32_t finish_agent(const PhysConsts &c, Agent &a, 
                                                  uint32_t *__restrict__ key_out)
 {
     solve_cube<SPRING>(c, a);
-    a.vx = clamp_sel(a.vx, -65504.0f, 65504.0f);
-    a.vy = clamp_sel(a.vy, -65504.0f, 65504.0f);
-    a.vz = clamp_sel(a.vz, -65504.0f, 65504.0f);
-    a.fx = clamp_sel(a.fx, -65504.0f, 65504.0f);
-    a.fy = clamp_sel(a.fy, -65504.0f, 65504.0f);
-    a.fz = clamp_sel(a.fz, -65504.0f, 65504.0f);
+    // clamp to the f16 range (physics.wgsl:82-83); with constant non-zero
+    // bounds v_max/v_min give exactly the compare-and-select result (a NaN
+    // operand yields the bound).
+    a.vx = fminf(fmaxf(a.vx, -65504.0f), 65504.0f);
+    a.vy = fminf(fmaxf(a.vy, -65504.0f), 65504.0f);
+    a.vz = fminf(fmaxf(a.vz, -65504.0f), 65504.0f);
+    a.fx = fminf(fmaxf(a.fx, -65504.0f), 65504.0f);
+    a.fy = fminf(fmaxf(a.fy, -65504.0f), 65504.0f);
+    a.fz = fminf(fmaxf(a.fz, -65504.0f), 65504.0f);
     pos_o[s] = make_float4(a.px, a.py, a.pz, a.scale);
     vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
                            pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
@@ -859,7 +900,7 @@ __device__ __forceinline__ void block_row_span(uint32_t lo, uint32_t hi, uint32_
 // and adds one atomic per row it touches (nearly sorted keys touch few rows,
 // so every row counter sees about as many atomics as blocks overlap it).
 __global__ __launch_bounds__(256) void row_count_kernel(const uint32_t *__restrict__ keys,
-                                                        uint32_t n, uint32_t grids,
+                                                        uint32_t n, uint32_t grids, FastDiv gdiv,
                                                         uint32_t *__restrict__ row_count)
 {
     __shared__ uint32_t s_hist[kMaxRows];
@@ -871,7 +912,7 @@ __global__ __launch_bounds__(256) void row_count_kernel(const uint32_t *__restri
 #pragma unroll
     for (uint32_t j = 0; j < kPartItems; ++j) {
         const uint32_t i = base + j * 256u + tid;
-        row[j] = i < n ? min(keys[i] / grids, grids - 1u) : kInvalid;
+        row[j] = i < n ? min(fast_div(keys[i], gdiv), grids - 1u) : kInvalid;
         if (i < n) {
             lo = min(lo, row[j]);
             hi = max(hi, row[j]);
@@ -893,7 +934,7 @@ __global__ __launch_bounds__(256) void row_count_kernel(const uint32_t *__restri
 }
 
 __global__ __launch_bounds__(256) void row_partition_kernel(
-    const uint32_t *__restrict__ keys, uint32_t n, uint32_t grids,
+    const uint32_t *__restrict__ keys, uint32_t n, uint32_t grids, FastDiv gdiv,
     const uint32_t *__restrict__ row_count, uint32_t *__restrict__ row_fill,
     uint32_t *__restrict__ row_start_out, uint64_t *__restrict__ pairs)
 {
@@ -920,7 +961,7 @@ __global__ __launch_bounds__(256) void row_partition_kernel(
         const uint32_t i = base + j * 256u + tid;
         key[j] = i < n ? keys[i] : 0u;
         if (i < n) {
-            const uint32_t r = min(key[j] / grids, rows - 1u);
+            const uint32_t r = min(fast_div(key[j], gdiv), rows - 1u);
             lo = min(lo, r);
             hi = max(hi, r);
         }
@@ -933,7 +974,7 @@ __global__ __launch_bounds__(256) void row_partition_kernel(
 #pragma unroll
     for (uint32_t j = 0; j < kPartItems; ++j) {
         const uint32_t i = base + j * 256u + tid;
-        const uint32_t r = min(key[j] / grids, rows - 1u);
+        const uint32_t r = min(fast_div(key[j], gdiv), rows - 1u);
         slot[j] = wave_slot_lds(s_mine, r, i < n);
     }
     __syncthreads();
@@ -947,7 +988,7 @@ __global__ __launch_bounds__(256) void row_partition_kernel(
     for (uint32_t j = 0; j < kPartItems; ++j) {
         const uint32_t i = base + j * 256u + tid;
         if (i < n) {
-            const uint32_t r = min(key[j] / grids, rows - 1u);
+            const uint32_t r = min(fast_div(key[j], gdiv), rows - 1u);
             const uint32_t cx = min(key[j] - r * grids, grids - 1u);
             const uint32_t dest = s_mine[r] + slot[j];
             if (dest < s_start[r + 1u]) pairs[dest] = ((uint64_t)cx << 32) | i;
@@ -1074,7 +1115,7 @@ __device__ __forceinline__ void collide_candidate(const PhysConsts &c, Agent &a,
     const float ty = a.py - bp.y;
     const float tz = a.pz - bp.z;
     const float d2 = tx * tx + ty * ty + tz * tz;
-    if (d2 <= (edge * edge) * 1.0001f) solve_sphere<SPRING>(c, a, bp, svfme, i);
+    if (d2 <= (edge * edge) * 1.0001f) sphere_contact<SPRING>(c, a, edge, tx, ty, tz, d2, svfme, i);
 }
 
 struct StageWindows {
@@ -1327,9 +1368,9 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, bool init, u
         return hipGetLastError();
     }
     hipLaunchKernelGGL(row_count_kernel, tiles, dim3(256), 0, stream, b.keys_in, n, G,
-                       b.row_count);
+                       c.grids_div, b.row_count);
     hipLaunchKernelGGL(row_partition_kernel, tiles, dim3(256), 0, stream, b.keys_in, n, G,
-                       b.row_count, b.row_fill, b.row_start, b.pairs);
+                       c.grids_div, b.row_count, b.row_fill, b.row_start, b.pairs);
     if ((e = mark(1)) != hipSuccess) return e;
     hipLaunchKernelGGL(row_sort_kernel, dim3(G), dim3(256), 0, stream, b.pairs, b.scratch_pairs,
                        b.row_start, b.row_count, b.row_fill, G, n, b.perm, b.off, b.pos_i,

@@ -302,3 +302,36 @@ def test_out_of_box_upload_is_clamped_like_oracle(gpu, oracle):
     r.upload(pos, vf)
     with g:
         compare_frames(g, r, 4)
+
+
+@pytest.mark.parametrize("grids,grid_size,n,mass_mode", [
+    (64, 3, 4096, "ones"),        # divisor 3: the general fast-division path
+    (100, 1, 10_000, "ones"),     # G not a power of two: row = key / 100
+    (64, 1, 4096, "random"),      # masses off powers of two: IEEE divisions
+    (48, 2, 2304, "pow2"),        # masses 0.5 / 2 / 4: exact reciprocals
+])
+def test_non_power_of_two_paths(gpu, oracle, grids, grid_size, n, mass_mode):
+    rng = np.random.default_rng(grids * 7 + grid_size)
+    ps = PhysicsSettings(number_of_grids_one_dimension=grids, grid_size=grid_size,
+                         half_map_height=6, e_agent=0.7, e_envir=0.6, time_delta=1 / 60,
+                         _max_number_of_agents=n)
+    hms = (grids * grid_size) // 2
+    pos = np.empty((n, 4), np.float32)
+    pos[:, 0] = rng.uniform(-hms + 0.5, hms - 0.5, n)
+    pos[:, 2] = rng.uniform(-hms + 0.5, hms - 0.5, n)
+    pos[:, 1] = rng.uniform(0.5, 11.5, n)
+    pos[:, 3] = rng.uniform(0.3, 0.7, n) if mass_mode == "random" else 0.5
+    if mass_mode == "random":
+        mass = rng.uniform(0.3, 3.0, n).astype(np.float32)
+    elif mass_mode == "pow2":
+        mass = rng.choice(np.float32([0.5, 1.0, 2.0, 4.0]), n)
+    else:
+        mass = np.ones(n, np.float32)
+    force = rng.uniform(-2, 2, (n, 3)).astype(np.float32)
+    vf = pack_vfme(rng.uniform(-1, 1, (n, 3)).astype(np.float32), force, mass)
+    g = Physics(ps)
+    g.upload_state(pos, vf)
+    r = oracle.World(ps)
+    r.upload(pos, vf)
+    with g:
+        compare_frames(g, r, 12)
