@@ -102,7 +102,7 @@ constexpr uint32_t kMaxRows = 4096;     // G <= 4096 (G^2 <= 2^24)
 constexpr uint32_t kPartItems = 4;
 constexpr uint32_t kPartTile = 256 * kPartItems;  // keys per partition block
 constexpr uint32_t kRowCap = 4096;      // spheres of one row sorted in LDS
-constexpr uint32_t kStageCap = 512;     // slots per staged neighbour-row window
+constexpr uint32_t kStageCap = 384;     // slots per staged neighbour-row window
 
 struct FrameBuffers {
     const uint32_t *keys_in;   // cell keys of the input state (unsorted)

@@ -478,14 +478,21 @@ __device__ __forceinline__ Agent agent_from(float4 p, uint4 q)
     return a;
 }
 
+// Neighbour velocity and mass as two words: vx | vy << 16 and vz | mass << 16
+// (the f16 halves of packed_vfme.x, .y low and .w low).
+__device__ __forceinline__ uint2 vel_mass_of(uint4 q)
+{
+    return make_uint2(q.x, (q.y & 0xFFFFu) | (q.w << 16));
+}
+
 // solve_sphere_constraint (physics.wgsl:300-331) from the offset
-// t = A.position - B.position, d2 = dot(t, t) and edge = r_B + r_A, against
-// the neighbour whose packed state is vfme[bo].  The neighbour's packed state
-// is read only when it is needed (contact, rigid response).
-template <bool SPRING>
+// t = A.position - B.position, d2 = dot(t, t) and edge = r_B + r_A.
+// `vel_mass()` yields the neighbour's velocity and mass; it is called only
+// for a contact with the rigid response.
+template <bool SPRING, typename VelMass>
 __device__ __forceinline__ void sphere_contact(const PhysConsts &c, Agent &a, float edge,
                                                float tx, float ty, float tz, float d2,
-                                               const uint4 *__restrict__ vfme, uint32_t bo)
+                                               VelMass vel_mass)
 {
     const float dist = sqrtf(d2);
     if (dist <= edge) {
@@ -493,10 +500,10 @@ __device__ __forceinline__ void sphere_contact(const PhysConsts &c, Agent &a, fl
         const float ny = ty / dist;
         const float nz = tz / dist;
         if (!SPRING) {
-            const uint4 q = vfme[bo];
+            const uint2 q = vel_mass();
             const float bvx = half_lo(q.x), bvy = half_hi(q.x), bvz = half_lo(q.y);
             const float m1 = a.m;
-            const float m2 = half_lo(q.w);
+            const float m2 = half_hi(q.y);
             const float rx = a.vx - bvx;
             const float ry = a.vy - bvy;
             const float rz = a.vz - bvz;
@@ -548,7 +555,8 @@ __device__ __forceinline__ void solve_sphere(const PhysConsts &c, Agent &a, floa
     const float tx = a.px - bp.x;
     const float ty = a.py - bp.y;
     const float tz = a.pz - bp.z;
-    sphere_contact<SPRING>(c, a, edge, tx, ty, tz, tx * tx + ty * ty + tz * tz, vfme, bo);
+    sphere_contact<SPRING>(c, a, edge, tx, ty, tz, tx * tx + ty * ty + tz * tz,
+                           [&]() { return vel_mass_of(vfme[bo]); });
 }
 
 template <bool SPRING>
@@ -1102,20 +1110,20 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
     }
 }
 
-template <bool SPRING>
+// solve_sphere_constraint behind a cheap conservative filter:
+// sqrt_rn(d2) <= edge implies d2 <= edge^2 (1 + 2^-23), far inside the 1e-4
+// margin used here, so every candidate that can be in contact reaches the
+// exact test.
+template <bool SPRING, typename VelMass>
 __device__ __forceinline__ void collide_candidate(const PhysConsts &c, Agent &a, float4 bp,
-                                                  const uint4 *__restrict__ svfme, uint32_t i)
+                                                  VelMass vel_mass)
 {
-    // Cheap conservative filter: sqrt_rn(d2) <= edge implies
-    // d2 <= edge^2 (1 + 2^-23), far inside the 1e-4 margin used here; the
-    // exact test of solve_sphere_constraint runs for every candidate that
-    // passes.
     const float edge = bp.w + a.scale;
     const float tx = a.px - bp.x;
     const float ty = a.py - bp.y;
     const float tz = a.pz - bp.z;
     const float d2 = tx * tx + ty * ty + tz * tz;
-    if (d2 <= (edge * edge) * 1.0001f) sphere_contact<SPRING>(c, a, edge, tx, ty, tz, d2, svfme, i);
+    if (d2 <= (edge * edge) * 1.0001f) sphere_contact<SPRING>(c, a, edge, tx, ty, tz, d2, vel_mass);
 }
 
 struct StageWindows {
@@ -1178,15 +1186,17 @@ template <bool SPRING, bool FROM_LDS>
 __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, uint32_t s,
                                                 const Ranges r,
                                                 const float4 *__restrict__ s_pos,
+                                                const uint2 *__restrict__ s_vm,
                                                 const float4 *__restrict__ spos,
                                                 const uint4 *__restrict__ svfme)
 {
     const uint32_t total = r.total;
-    auto candidate = [&](uint32_t j, uint32_t &i) -> float4 {
-        const bool b1 = j >= r.n0, b2 = j >= r.n01;
-        i = j + r.slot0 + (b1 ? r.slot1 : 0u) + (b2 ? r.slot2 : 0u);
-        if (FROM_LDS) return s_pos[j + r.lds0 + (b1 ? r.lds1 : 0u) + (b2 ? r.lds2 : 0u)];
-        return spos[i];
+    // sorted slot and LDS index of candidate j
+    auto slot_of = [&](uint32_t j) {
+        return j + r.slot0 + (j >= r.n0 ? r.slot1 : 0u) + (j >= r.n01 ? r.slot2 : 0u);
+    };
+    auto lds_of = [&](uint32_t j) {
+        return j + r.lds0 + (j >= r.n0 ? r.lds1 : 0u) + (j >= r.n01 ? r.lds2 : 0u);
     };
     const float margin = 0.25f * a.scale;
     const float still = (margin * 0.999f) * (margin * 0.999f);
@@ -1196,8 +1206,8 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
         const float rx = a.px, ry = a.py, rz = a.pz;
         uint64_t near = 0;
         for (uint32_t t = j; t < end; ++t) {
-            uint32_t i;
-            const float4 bp = candidate(t, i);
+            const uint32_t i = slot_of(t);
+            const float4 bp = FROM_LDS ? s_pos[lds_of(t)] : spos[i];
             const float reach = (bp.w + a.scale) + margin;
             const float dx = rx - bp.x, dy = ry - bp.y, dz = rz - bp.z;
             const float d2 = dx * dx + dy * dy + dz * dz;
@@ -1207,13 +1217,17 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
         while (near) {
             const uint32_t t = j + (uint32_t)__builtin_ctzll(near);
             near &= near - 1ull;
-            uint32_t i;
-            const float4 bp = candidate(t, i);
-            collide_candidate<SPRING>(c, a, bp, svfme, i);
+            if (FROM_LDS) {
+                const uint32_t l = lds_of(t);
+                collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
+            } else {
+                const uint32_t i = slot_of(t);
+                collide_candidate<SPRING>(c, a, spos[i], [&]() { return vel_mass_of(svfme[i]); });
+            }
             const float ex = a.px - rx, ey = a.py - ry, ez = a.pz - rz;
             if (!(ex * ex + ey * ey + ez * ez <= still)) {
-                next = t + 1u;
-                break;
+                next = t + 1u;  // moved beyond the margin: rescan after t
+                near = 0;
             }
         }
         j = next;
@@ -1234,6 +1248,7 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     uint32_t n)
 {
     __shared__ float4 s_pos[3 * kStageCap];  // the three windows back to back
+    __shared__ uint2 s_vm[3 * kStageCap];    // their velocity and mass halves
     __shared__ StageWindows s_win;
     __shared__ uint32_t s_kfirst, s_klast;
 
@@ -1287,7 +1302,10 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
 #pragma unroll
         for (uint32_t w = 0; w < 3; ++w) {
             const uint32_t lo = s_win.lo[w], cnt = s_win.cnt[w];
-            for (uint32_t j = tid; j < cnt; j += 256u) s_pos[w * kStageCap + j] = spos[lo + j];
+            for (uint32_t j = tid; j < cnt; j += 256u) {
+                s_pos[w * kStageCap + j] = spos[lo + j];
+                if (!SPRING) s_vm[w * kStageCap + j] = vel_mass_of(svfme[lo + j]);
+            }
         }
     }
     __syncthreads();
@@ -1298,9 +1316,9 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     verlet(c, a);
     const float ox = a.px, oy = a.py, oz = a.pz;
     if (staged)
-        walk_candidates<SPRING, true>(c, a, s, rg, s_pos, spos, svfme);
+        walk_candidates<SPRING, true>(c, a, s, rg, s_pos, s_vm, spos, svfme);
     else
-        walk_candidates<SPRING, false>(c, a, s, rg, s_pos, spos, svfme);
+        walk_candidates<SPRING, false>(c, a, s, rg, s_pos, s_vm, spos, svfme);
     jitter(a, s, ox, oy, oz);
     (void)finish_agent<SPRING>(c, a, s, pos_o, vfme_o, key_out);
 }
