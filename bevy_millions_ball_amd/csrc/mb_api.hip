@@ -140,9 +140,7 @@ struct mb_physics {
     uint32_t *off = nullptr;        // per-cell slot offsets, N + 1
     uint32_t *rows = nullptr;       // row_count | row_fill | row_start
     uint64_t *pairs = nullptr;      // N
-    uint64_t *scratch_pairs = nullptr;  // N
-    float4 *spos = nullptr;         // N: input state gathered into sorted order
-    uint4 *svfme = nullptr;         // N
+    uint32_t *scratch_idx = nullptr;    // N
     // Profiling: four events per frame (before the row partition, before the
     // row sort, before collide + integrate, after it) recorded without
     // blocking; read back by mb_physics_profile_read.
@@ -191,9 +189,7 @@ static void release(mb_physics *p)
     if (p->off) (void)hipFree(p->off);
     if (p->rows) (void)hipFree(p->rows);
     if (p->pairs) (void)hipFree(p->pairs);
-    if (p->scratch_pairs) (void)hipFree(p->scratch_pairs);
-    if (p->spos) (void)hipFree(p->spos);
-    if (p->svfme) (void)hipFree(p->svfme);
+    if (p->scratch_idx) (void)hipFree(p->scratch_idx);
     for (auto &fe : p->prof_events)
         for (hipEvent_t e : fe)
             if (e) (void)hipEventDestroy(e);
@@ -243,9 +239,7 @@ int mb_physics_create(const mb_settings *settings, mb_physics **out)
     ok = ok && hipMalloc(&p->off, (n + 1) * 4u) == hipSuccess;
     ok = ok && hipMalloc(&p->rows, row_words * 4u) == hipSuccess;
     ok = ok && hipMalloc(&p->pairs, n * 8u) == hipSuccess;
-    ok = ok && hipMalloc(&p->scratch_pairs, n * 8u) == hipSuccess;
-    ok = ok && hipMalloc(&p->spos, n * sizeof(float4)) == hipSuccess;
-    ok = ok && hipMalloc(&p->svfme, n * sizeof(uint4)) == hipSuccess;
+    ok = ok && hipMalloc(&p->scratch_idx, n * 4u) == hipSuccess;
     if (!ok) {
         release(p);
         delete p;
@@ -387,9 +381,7 @@ static int run_frame(mb_physics *p, hipStream_t st)
     b.row_fill = p->row_fill();
     b.row_start = p->row_start();
     b.pairs = p->pairs;
-    b.scratch_pairs = p->scratch_pairs;
-    b.spos = p->spos;
-    b.svfme = p->svfme;
+    b.scratch_idx = p->scratch_idx;
     b.pos_i = p->pos[par];
     b.vfme_i = p->vfme[par];
     b.pos_o = p->pos[q];

@@ -113,9 +113,7 @@ struct FrameBuffers {
     uint32_t *row_fill;        // kMaxRows: partition cursors
     uint32_t *row_start;       // kMaxRows + 1: exclusive scan of row_count
     uint64_t *pairs;           // n: (cx, index) partitioned by row
-    uint64_t *scratch_pairs;   // n: rows larger than kRowCap
-    float4 *spos;              // n: input positions gathered into sorted order
-    uint4 *svfme;              // n: input packed_vfme gathered into sorted order
+    uint32_t *scratch_idx;     // n: rows larger than kRowCap
     const float4 *pos_i;
     const uint4 *vfme_i;
     float4 *pos_o;

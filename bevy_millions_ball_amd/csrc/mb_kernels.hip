@@ -816,11 +816,9 @@ hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
 //   row_sort_kernel       one block per row: counting sort by cx in LDS, then
 //                         every pair is ranked among the pairs of its cell by
 //                         index, which restores argsort_stable exactly; it
-//                         writes perm, the per-cell slot offsets and the input
-//                         state gathered into sorted order;
-//   physics_step_kernel   collide + integrate on the sorted state with the
-//                         neighbour-row windows staged in LDS; it writes the
-//                         next keys.
+//                         writes perm and the per-cell slot offsets;
+//   physics_step_kernel   collide + integrate with the neighbour-row windows
+//                         gathered into LDS; it writes the next keys.
 // Global atomics are limited to one per (block, row touched): many waves
 // adding to one counter serialise badly on this part.
 // No stage needs a cross-block ordering: there is no look-back and no spin.
@@ -1004,16 +1002,17 @@ __global__ __launch_bounds__(256) void row_partition_kernel(
     }
 }
 
+// One block per row.  Dynamic LDS: `grids` cell counters followed by
+// kRowCap indices.
 __global__ __launch_bounds__(256) void row_sort_kernel(
-    const uint64_t *__restrict__ pairs, uint64_t *__restrict__ scratch_pairs,
+    const uint64_t *__restrict__ pairs, uint32_t *__restrict__ scratch_idx,
     const uint32_t *__restrict__ row_start, uint32_t *__restrict__ row_count,
     uint32_t *__restrict__ row_fill, uint32_t grids, uint32_t n,
-    uint32_t *__restrict__ perm, uint32_t *__restrict__ off,
-    const float4 *__restrict__ pos_i, const uint4 *__restrict__ vfme_i,
-    float4 *__restrict__ spos, uint4 *__restrict__ svfme)
+    uint32_t *__restrict__ perm, uint32_t *__restrict__ off)
 {
-    __shared__ uint32_t s_cell[kMaxRows];
-    __shared__ uint64_t s_el[kRowCap];
+    extern __shared__ uint32_t s_dyn[];
+    uint32_t *s_cell = s_dyn;           // grids words
+    uint32_t *s_idx = s_dyn + grids;    // kRowCap words
     __shared__ uint32_t s_scratch[4];
     const uint32_t tid = threadIdx.x;
     const uint32_t r = xcd_block(blockIdx.x, gridDim.x);
@@ -1042,11 +1041,11 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) {
             const uint32_t e = j * 256u + tid;
-            if (e < m) s_el[atomicAdd(&s_cell[(uint32_t)(el[j] >> 32)], 1u)] = el[j];
+            if (e < m) s_idx[atomicAdd(&s_cell[(uint32_t)(el[j] >> 32)], 1u)] = (uint32_t)el[j];
         }
         __syncthreads();
-        // s_cell[c] is now the end of cell c inside the row; rank each pair
-        // among its cell's pairs by input index.
+        // s_cell[c] is now the end of cell c inside the row; rank each index
+        // among its cell's indices.
         uint32_t dst[kPer];
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) {
@@ -1058,25 +1057,19 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
                 const uint32_t cs = c ? s_cell[c - 1u] : 0u;
                 const uint32_t ce = s_cell[c];
                 uint32_t rank = 0;
-                for (uint32_t q = cs; q < ce; ++q) rank += (uint32_t)s_el[q] < idx;
+                for (uint32_t q = cs; q < ce; ++q) rank += s_idx[q] < idx;
                 dst[j] = cs + rank;
             }
         }
         __syncthreads();
-        uint32_t *s_perm = reinterpret_cast<uint32_t *>(s_el);
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j)
-            if (dst[j] != kInvalid) s_perm[dst[j]] = (uint32_t)el[j];
+            if (dst[j] != kInvalid) s_idx[dst[j]] = (uint32_t)el[j];
         __syncthreads();
-        for (uint32_t e = tid; e < m; e += 256u) {
-            const uint32_t idx = s_perm[e];
-            perm[base + e] = idx;
-            spos[base + e] = pos_i[idx];
-            svfme[base + e] = vfme_i[idx];
-        }
+        for (uint32_t e = tid; e < m; e += 256u) perm[base + e] = s_idx[e];
     } else {
         // A row larger than the LDS tile: the same steps with the scattered
-        // copy in global scratch.
+        // indices in global scratch.
         for (uint32_t e = tid; e < m; e += 256u)
             atomicAdd(&s_cell[(uint32_t)(pairs[base + e] >> 32)], 1u);
         __syncthreads();
@@ -1085,7 +1078,7 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
         __syncthreads();
         for (uint32_t e = tid; e < m; e += 256u) {
             const uint64_t v = pairs[base + e];
-            scratch_pairs[base + atomicAdd(&s_cell[(uint32_t)(v >> 32)], 1u)] = v;
+            scratch_idx[base + atomicAdd(&s_cell[(uint32_t)(v >> 32)], 1u)] = (uint32_t)v;
         }
         __threadfence_block();
         __syncthreads();
@@ -1096,11 +1089,8 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
             const uint32_t cs = c ? s_cell[c - 1u] : 0u;
             const uint32_t ce = s_cell[c];
             uint32_t rank = 0;
-            for (uint32_t q = cs; q < ce; ++q) rank += (uint32_t)scratch_pairs[base + q] < idx;
-            const uint32_t slot = base + cs + rank;
-            perm[slot] = idx;
-            spos[slot] = pos_i[idx];
-            svfme[slot] = vfme_i[idx];
+            for (uint32_t q = cs; q < ce; ++q) rank += scratch_idx[base + q] < idx;
+            perm[base + cs + rank] = idx;
         }
     }
     if (tid == 0) {
@@ -1146,8 +1136,8 @@ struct StageWindows {
 // most of the wave divergence of the contact path.
 //
 // FROM_LDS reads the neighbour positions from the staged windows (every
-// range lies inside its window), otherwise from the sorted state in global
-// memory.  The two variants are separate code paths so that the compiler
+// range lies inside its window), otherwise from the input state in global
+// memory through perm.  The two variants are separate code paths so that the compiler
 // never merges the LDS and global reads into one flat access.
 // A sphere's candidates, numbered j = 0 .. total-1 over its middle, bottom
 // and top ranges in that order: candidate j is sorted slot
@@ -1187,8 +1177,9 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
                                                 const Ranges r,
                                                 const float4 *__restrict__ s_pos,
                                                 const uint2 *__restrict__ s_vm,
-                                                const float4 *__restrict__ spos,
-                                                const uint4 *__restrict__ svfme)
+                                                const uint32_t *__restrict__ perm,
+                                                const float4 *__restrict__ pos_i,
+                                                const uint4 *__restrict__ vfme_i)
 {
     const uint32_t total = r.total;
     // sorted slot and LDS index of candidate j
@@ -1207,7 +1198,7 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
         uint64_t near = 0;
         for (uint32_t t = j; t < end; ++t) {
             const uint32_t i = slot_of(t);
-            const float4 bp = FROM_LDS ? s_pos[lds_of(t)] : spos[i];
+            const float4 bp = FROM_LDS ? s_pos[lds_of(t)] : pos_i[perm[i]];
             const float reach = (bp.w + a.scale) + margin;
             const float dx = rx - bp.x, dy = ry - bp.y, dz = rz - bp.z;
             const float d2 = dx * dx + dy * dy + dz * dz;
@@ -1221,8 +1212,8 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
                 const uint32_t l = lds_of(t);
                 collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
             } else {
-                const uint32_t i = slot_of(t);
-                collide_candidate<SPRING>(c, a, spos[i], [&]() { return vel_mass_of(svfme[i]); });
+                const uint32_t bo = perm[slot_of(t)];
+                collide_candidate<SPRING>(c, a, pos_i[bo], [&]() { return vel_mass_of(vfme_i[bo]); });
             }
             const float ex = a.px - rx, ey = a.py - ry, ez = a.pz - rz;
             if (!(ex * ex + ey * ey + ez * ez <= still)) {
@@ -1234,17 +1225,16 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
     }
 }
 
-// main() with LOOP_PIPELINE for a block of 256 consecutive sorted slots, on
-// the state already gathered into sorted order.  The slots' cells are
-// non-decreasing, so the candidates of the whole block in each of the three
-// rows form one contiguous slot window; the windows are copied to LDS once,
-// and each thread walks its own three ranges (physics.wgsl:119-206) in the
-// shader's order.
+// main() with LOOP_PIPELINE for a block of 256 consecutive sorted slots.
+// The slots' cells are non-decreasing, so the candidates of the whole block
+// in each of the three rows form one contiguous slot window; the windows are
+// gathered into LDS once (through perm), and each thread walks its own three
+// ranges (physics.wgsl:119-206) in the shader's order.
 template <bool SPRING>
 __global__ __launch_bounds__(256) void physics_step_kernel(
-    PhysConsts c, const uint32_t *__restrict__ off, uint32_t *__restrict__ key_out,
-    const float4 *__restrict__ spos,
-    const uint4 *__restrict__ svfme, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
+    PhysConsts c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
+    uint32_t *__restrict__ key_out, const float4 *__restrict__ pos_i,
+    const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
     uint32_t n)
 {
     __shared__ float4 s_pos[3 * kStageCap];  // the three windows back to back
@@ -1261,7 +1251,8 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     const uint32_t G = c.grids;
     const uint32_t C = G * G;
 
-    Agent a = agent_from(spos[s], svfme[s]);
+    const uint32_t o = perm[s];
+    Agent a = agent_from(pos_i[o], vfme_i[o]);
     uint32_t cx, cz;
     cell_of(c, a.px, a.pz, cx, cz);
     const uint32_t k = cz * G + cx;
@@ -1303,8 +1294,9 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
         for (uint32_t w = 0; w < 3; ++w) {
             const uint32_t lo = s_win.lo[w], cnt = s_win.cnt[w];
             for (uint32_t j = tid; j < cnt; j += 256u) {
-                s_pos[w * kStageCap + j] = spos[lo + j];
-                if (!SPRING) s_vm[w * kStageCap + j] = vel_mass_of(svfme[lo + j]);
+                const uint32_t bo = perm[lo + j];
+                s_pos[w * kStageCap + j] = pos_i[bo];
+                if (!SPRING) s_vm[w * kStageCap + j] = vel_mass_of(vfme_i[bo]);
             }
         }
     }
@@ -1316,9 +1308,9 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     verlet(c, a);
     const float ox = a.px, oy = a.py, oz = a.pz;
     if (staged)
-        walk_candidates<SPRING, true>(c, a, s, rg, s_pos, s_vm, spos, svfme);
+        walk_candidates<SPRING, true>(c, a, s, rg, s_pos, s_vm, perm, pos_i, vfme_i);
     else
-        walk_candidates<SPRING, false>(c, a, s, rg, s_pos, s_vm, spos, svfme);
+        walk_candidates<SPRING, false>(c, a, s, rg, s_pos, s_vm, perm, pos_i, vfme_i);
     jitter(a, s, ox, oy, oz);
     (void)finish_agent<SPRING>(c, a, s, pos_o, vfme_o, key_out);
 }
@@ -1390,16 +1382,16 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, bool init, u
     hipLaunchKernelGGL(row_partition_kernel, tiles, dim3(256), 0, stream, b.keys_in, n, G,
                        c.grids_div, b.row_count, b.row_fill, b.row_start, b.pairs);
     if ((e = mark(1)) != hipSuccess) return e;
-    hipLaunchKernelGGL(row_sort_kernel, dim3(G), dim3(256), 0, stream, b.pairs, b.scratch_pairs,
-                       b.row_start, b.row_count, b.row_fill, G, n, b.perm, b.off, b.pos_i,
-                       b.vfme_i, b.spos, b.svfme);
+    const size_t sort_lds = ((size_t)G + kRowCap) * 4u;
+    hipLaunchKernelGGL(row_sort_kernel, dim3(G), dim3(256), sort_lds, stream, b.pairs,
+                       b.scratch_idx, b.row_start, b.row_count, b.row_fill, G, n, b.perm, b.off);
     if ((e = mark(2)) != hipSuccess) return e;
     if (c.enable_spring) {
-        hipLaunchKernelGGL(physics_step_kernel<true>, grid, dim3(256), 0, stream, c, b.off,
-                           b.keys_out, b.spos, b.svfme, b.pos_o, b.vfme_o, n);
+        hipLaunchKernelGGL(physics_step_kernel<true>, grid, dim3(256), 0, stream, c, b.perm, b.off,
+                           b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, n);
     } else {
-        hipLaunchKernelGGL(physics_step_kernel<false>, grid, dim3(256), 0, stream, c, b.off,
-                           b.keys_out, b.spos, b.svfme, b.pos_o, b.vfme_o, n);
+        hipLaunchKernelGGL(physics_step_kernel<false>, grid, dim3(256), 0, stream, c, b.perm,
+                           b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, n);
     }
     if ((e = mark(3)) != hipSuccess) return e;
     return hipGetLastError();
