@@ -816,7 +816,8 @@ hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
 //   row_sort_kernel       one block per row: counting sort by cx in LDS, then
 //                         every pair is ranked among the pairs of its cell by
 //                         index, which restores argsort_stable exactly; it
-//                         writes perm and the per-cell slot offsets;
+//                         writes perm, the sorted keys and the per-cell slot
+//                         offsets;
 //   physics_step_kernel   collide + integrate with the neighbour-row windows
 //                         gathered into LDS; it writes the next keys.
 // Global atomics are limited to one per (block, row touched): many waves
@@ -1008,7 +1009,7 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
     const uint64_t *__restrict__ pairs, uint32_t *__restrict__ scratch_idx,
     const uint32_t *__restrict__ row_start, uint32_t *__restrict__ row_count,
     uint32_t *__restrict__ row_fill, uint32_t grids, uint32_t n,
-    uint32_t *__restrict__ perm, uint32_t *__restrict__ off)
+    uint32_t *__restrict__ perm, uint32_t *__restrict__ off, uint32_t *__restrict__ sorted_keys)
 {
     extern __shared__ uint32_t s_dyn[];
     uint32_t *s_cell = s_dyn;           // grids words
@@ -1067,6 +1068,11 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
             if (dst[j] != kInvalid) s_idx[dst[j]] = (uint32_t)el[j];
         __syncthreads();
         for (uint32_t e = tid; e < m; e += 256u) perm[base + e] = s_idx[e];
+        // sorted keys: the cell of slot base + e is the cell whose end exceeds e
+        for (uint32_t c = tid; c < grids; c += 256u) {
+            const uint32_t cs = c ? s_cell[c - 1u] : 0u;
+            for (uint32_t e = cs; e < s_cell[c]; ++e) sorted_keys[base + e] = r * grids + c;
+        }
     } else {
         // A row larger than the LDS tile: the same steps with the scattered
         // indices in global scratch.
@@ -1091,6 +1097,7 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
             uint32_t rank = 0;
             for (uint32_t q = cs; q < ce; ++q) rank += scratch_idx[base + q] < idx;
             perm[base + cs + rank] = idx;
+            sorted_keys[base + cs + rank] = r * grids + c;
         }
     }
     if (tid == 0) {
@@ -1115,11 +1122,6 @@ __device__ __forceinline__ void collide_candidate(const PhysConsts &c, Agent &a,
     const float d2 = tx * tx + ty * ty + tz * tz;
     if (d2 <= (edge * edge) * 1.0001f) sphere_contact<SPRING>(c, a, edge, tx, ty, tz, d2, vel_mass);
 }
-
-struct StageWindows {
-    uint32_t lo[3];
-    uint32_t cnt[3];
-};
 
 // Walks the candidates of one sphere: middle, bottom and top ranges in the
 // shader's order (physics.wgsl:141-147, 200-205), skipping its own slot.
@@ -1229,18 +1231,18 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
 // The slots' cells are non-decreasing, so the candidates of the whole block
 // in each of the three rows form one contiguous slot window; the windows are
 // gathered into LDS once (through perm), and each thread walks its own three
-// ranges (physics.wgsl:119-206) in the shader's order.
+// ranges (physics.wgsl:119-206) in the shader's order.  The block's first and
+// last cells come from the sorted keys, so the window bounds are uniform
+// loads issued at launch and one barrier separates staging from the walk.
 template <bool SPRING>
 __global__ __launch_bounds__(256) void physics_step_kernel(
     PhysConsts c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
-    uint32_t *__restrict__ key_out, const float4 *__restrict__ pos_i,
+    uint32_t *__restrict__ keys, const float4 *__restrict__ pos_i,
     const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
     uint32_t n)
 {
     __shared__ float4 s_pos[3 * kStageCap];  // the three windows back to back
     __shared__ uint2 s_vm[3 * kStageCap];    // their velocity and mass halves
-    __shared__ StageWindows s_win;
-    __shared__ uint32_t s_kfirst, s_klast;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
@@ -1251,14 +1253,31 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     const uint32_t G = c.grids;
     const uint32_t C = G * G;
 
+    // `keys` holds the sorted keys on entry (the cell of each slot's sphere,
+    // computed from the same position by the previous frame) and receives
+    // the next keys; every slot is read and then written by its own thread.
+    const uint32_t k = keys[s];
+    const uint32_t kfirst = keys[s0];
+    const uint32_t klast = keys[s0 + last];
     const uint32_t o = perm[s];
     Agent a = agent_from(pos_i[o], vfme_i[o]);
-    uint32_t cx, cz;
-    cell_of(c, a.px, a.pz, cx, cz);
-    const uint32_t k = cz * G + cx;
-    if (tid == 0) s_kfirst = k;
-    if (tid == last) s_klast = k;
+    const uint32_t cz = fast_div(k, c.grids_div);
+    const uint32_t cx = k - cz * G;
 
+    // windows: every slot of cells [kfirst + dk - 1, klast + dk + 1] for
+    // dk = 0, -G, +G, a superset of each thread's range in that row
+    uint32_t wlo[3], wcnt[3];
+#pragma unroll
+    for (uint32_t w = 0; w < 3; ++w) {
+        const int64_t dk = w == 0 ? 0 : (w == 1 ? -(int64_t)G : (int64_t)G);
+        int64_t lc = (int64_t)kfirst + dk - 1;
+        int64_t hc = (int64_t)klast + dk + 2;
+        lc = lc < 0 ? 0 : (lc > (int64_t)C ? (int64_t)C : lc);
+        hc = hc < 0 ? 0 : (hc > (int64_t)C ? (int64_t)C : hc);
+        const uint32_t lo = off[lc];
+        wlo[w] = lo;
+        wcnt[w] = max(off[hc], lo) - lo;
+    }
     // own ranges of physics.wgsl:123-160 as half-open slot intervals
     const uint32_t lft = cx > 0u ? 1u : 0u;
     const uint32_t rgt = cx < G - 1u ? 2u : 1u;
@@ -1272,29 +1291,12 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
         lo2 = off[k + G - lft];
         hi2 = off[k + G + rgt];
     }
-    __syncthreads();
-    if (tid < 3u) {
-        // window w: every slot of cells [kfirst + dk - 1, klast + dk + 1],
-        // a superset of each thread's range in that row
-        const int64_t dk = tid == 0 ? 0 : (tid == 1 ? -(int64_t)G : (int64_t)G);
-        int64_t lc = (int64_t)s_kfirst + dk - 1;
-        int64_t hc = (int64_t)s_klast + dk + 2;
-        lc = lc < 0 ? 0 : (lc > (int64_t)C ? (int64_t)C : lc);
-        hc = hc < 0 ? 0 : (hc > (int64_t)C ? (int64_t)C : hc);
-        const uint32_t lo = off[lc];
-        const uint32_t hi = max(off[hc], lo);
-        s_win.lo[tid] = lo;
-        s_win.cnt[tid] = hi - lo;
-    }
-    __syncthreads();
-    const bool staged = s_win.cnt[0] <= kStageCap && s_win.cnt[1] <= kStageCap &&
-                        s_win.cnt[2] <= kStageCap;
+    const bool staged = wcnt[0] <= kStageCap && wcnt[1] <= kStageCap && wcnt[2] <= kStageCap;
     if (staged) {
 #pragma unroll
         for (uint32_t w = 0; w < 3; ++w) {
-            const uint32_t lo = s_win.lo[w], cnt = s_win.cnt[w];
-            for (uint32_t j = tid; j < cnt; j += 256u) {
-                const uint32_t bo = perm[lo + j];
+            for (uint32_t j = tid; j < wcnt[w]; j += 256u) {
+                const uint32_t bo = perm[wlo[w] + j];
                 s_pos[w * kStageCap + j] = pos_i[bo];
                 if (!SPRING) s_vm[w * kStageCap + j] = vel_mass_of(vfme_i[bo]);
             }
@@ -1303,8 +1305,7 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     __syncthreads();
     if (!valid) return;
 
-    const Ranges rg =
-        make_ranges(lo0, hi0, lo1, hi1, lo2, hi2, s_win.lo[0], s_win.lo[1], s_win.lo[2]);
+    const Ranges rg = make_ranges(lo0, hi0, lo1, hi1, lo2, hi2, wlo[0], wlo[1], wlo[2]);
     verlet(c, a);
     const float ox = a.px, oy = a.py, oz = a.pz;
     if (staged)
@@ -1312,7 +1313,7 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     else
         walk_candidates<SPRING, false>(c, a, s, rg, s_pos, s_vm, perm, pos_i, vfme_i);
     jitter(a, s, ox, oy, oz);
-    (void)finish_agent<SPRING>(c, a, s, pos_o, vfme_o, key_out);
+    (void)finish_agent<SPRING>(c, a, s, pos_o, vfme_o, keys);
 }
 
 // INIT frame of the frame path: the sort of the all-zero keys is the
@@ -1384,7 +1385,8 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, bool init, u
     if ((e = mark(1)) != hipSuccess) return e;
     const size_t sort_lds = ((size_t)G + kRowCap) * 4u;
     hipLaunchKernelGGL(row_sort_kernel, dim3(G), dim3(256), sort_lds, stream, b.pairs,
-                       b.scratch_idx, b.row_start, b.row_count, b.row_fill, G, n, b.perm, b.off);
+                       b.scratch_idx, b.row_start, b.row_count, b.row_fill, G, n, b.perm, b.off,
+                       b.keys_out);
     if ((e = mark(2)) != hipSuccess) return e;
     if (c.enable_spring) {
         hipLaunchKernelGGL(physics_step_kernel<true>, grid, dim3(256), 0, stream, c, b.perm, b.off,
