@@ -33,18 +33,34 @@ HBM_PEAK_GBS = 8000.0
 
 def physics_bytes_per_sphere(n: int, cells: int) -> float:
     """Algorithmic HBM bytes of one collide + integrate launch per sphere:
-    perm 4 + state in 32 + cell offsets 4 * C / N + state out 32 + key out 4."""
-    return 72.0 + 4.0 * cells / n
+    sorted key in 4 + perm 4 + state in 32 + cell offsets 4 * C / N
+    + state out 32 + next key out 4 (written over the sorted key)."""
+    return 76.0 + 4.0 * cells / n
 
 
 def partition_bytes_per_sphere() -> float:
-    """Row partition: key in 4 + (column, index) pair out 8."""
-    return 12.0
+    """Row count (key in 4) + row partition (key in 4 + (column, index) pair
+    out 8): the first timed stage."""
+    return 16.0
 
 
 def row_sort_bytes_per_sphere(n: int, cells: int) -> float:
-    """Row sort: pair in 8 + perm out 4 + cell offsets out 4 * C / N."""
-    return 12.0 + 4.0 * cells / n
+    """Row sort: pair in 8 + perm out 4 + sorted key out 4 + cell offsets
+    out 4 * C / N."""
+    return 16.0 + 4.0 * cells / n
+
+
+def read_traffic(name: str):
+    """HBM bytes per collide + integrate launch from the counter summary
+    profiles/traffic_<workload>.txt (`key = value` lines), or None."""
+    path = ROOT / "profiles" / f"traffic_{name}.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        key, sep, val = line.partition("=")
+        if sep and key.strip() == "physics_step_kernel_bytes_per_launch":
+            return float(val.strip())
+    return None
 
 
 def cpu_baseline(cfg, pos, vf, budget_s: float, threads: int) -> dict:
@@ -135,10 +151,7 @@ def main() -> int:
     part_gbs = partition_bytes_per_sphere() * n / (stage_ms[0] * 1e-3) / 1e9
     rsort_gbs = row_sort_bytes_per_sphere(n, cells) * n / (stage_ms[1] * 1e-3) / 1e9
 
-    traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{cfg.name}.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("physics_loop_bytes_per_launch")
+    traffic = read_traffic(cfg.name)
 
     line = {
         "metric": METRIC,
