@@ -115,7 +115,9 @@ def main() -> int:
     pos, vf = generate(cfg, seed=args.seed)
     torch.cuda.set_device(0)
     phys = Physics(cfg.settings())
-    phys.upload_state(pos, vf)
+    t0 = time.perf_counter()
+    phys.upload_state(pos, vf)  # host -> HBM, synchronised by the library
+    upload_ms = (time.perf_counter() - t0) * 1e3
     phys.run(1)  # INIT frame
     phys.run(args.warmup)
     torch.cuda.synchronize()
@@ -144,6 +146,9 @@ def main() -> int:
         torch.cuda.synchronize()
         lat.append((time.perf_counter() - t) * 1e3)
     lat = np.array(lat)
+    t0 = time.perf_counter()
+    phys.download_state()  # HBM -> host, synchronised by the library
+    download_ms = (time.perf_counter() - t0) * 1e3
     phys.close()
 
     phys_bytes = physics_bytes_per_sphere(n, cells) * n
@@ -183,6 +188,8 @@ def main() -> int:
         },
         "p50_step_ms": float(np.percentile(lat, 50)),
         "p99_step_ms": float(np.percentile(lat, 99)),
+        "upload_ms": upload_ms,
+        "download_ms": download_ms,
         "device_step_ms_p50": float(np.percentile(device_step_ms, 50)),
         "device_step_ms_p99": float(np.percentile(device_step_ms, 99)),
         "stage_ms": {"row_partition": float(stage_ms[0]), "row_sort": float(stage_ms[1]),

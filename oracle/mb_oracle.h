@@ -18,6 +18,9 @@
  *   - u32(f32) truncates toward zero and saturates, NaN -> 0;
  *   - the cell coordinate is clamped to [0, G-1] (the reference indexes out of
  *     bounds there; in-box positions never reach the clamp).
+ * The physics pass is pinned by closed forms and invariants; at the level of
+ * a wgpu adapter's own rounding of physics.wgsl it is parity unpinned (no
+ * adapter run is available to compare with).
  */
 #ifndef MB_ORACLE_H
 #define MB_ORACLE_H
