@@ -88,6 +88,102 @@ def cpu_baseline(cfg, pos, vf, budget_s: float, threads: int) -> dict:
     }
 
 
+def serving_bench(n_requests: int, n_poisson: int, seed: int) -> dict:
+    """Batched serving of the mixed-scene workload against stepping the same
+    requests one by one on the same GPU, then latency under Poisson load.
+
+    Requests: N log-uniform in [4096, 1,048,576], G = next pow2 of ceil(sqrt N),
+    1-60 LOOP steps, plain-step distributions (bevy_millions_ball_amd.serving).
+    The timed region of both throughput legs is the server's work after the
+    requests were handed over: admission, host->device upload, INIT + LOOP
+    frames, device->host retirement.  Throughput counts LOOP sphere-steps."""
+    from bevy_millions_ball_amd.serving import Server, poisson_requests, request_scene
+
+    reqs = poisson_requests(n_requests, rate_per_s=200.0, seed=seed)
+    scenes = [request_scene(r) for r in reqs]
+    work = float(sum(r.number_of_agents * r.steps for r in reqs))
+
+    # warm up the kernels and the pinned pool on one small scene
+    with Server() as srv:
+        st, pos, vf = scenes[0]
+        rid = srv.submit(st, pos, vf, 2)
+        srv.drain()
+        srv.collect(rid)
+
+    with Server(profile_frames=True) as srv:
+        ids = [srv.submit(st, pos, vf, r.steps) for (st, pos, vf), r in zip(scenes, reqs)]
+        t0 = time.perf_counter()
+        srv.drain()
+        batched_s = time.perf_counter() - t0
+        frames_batched = srv.stats()["frames"]
+        batched_device_s = float(srv.frame_times().sum()) * 1e-3
+        for i in ids:
+            srv.collect(i)
+
+    one_s = 0.0
+    with Server() as srv:
+        for (st, pos, vf), r in zip(scenes, reqs):
+            rid = srv.submit(st, pos, vf, r.steps)
+            t0 = time.perf_counter()
+            srv.drain()
+            one_s += time.perf_counter() - t0
+            srv.collect(rid)
+        frames_one = srv.stats()["frames"]
+
+    # Poisson arrivals at 200/s: submit at arrival time, tick while work remains
+    preqs = poisson_requests(n_poisson, rate_per_s=200.0, seed=seed + 1)
+    pscenes = [request_scene(r) for r in preqs]
+    lat = []
+    with Server(profile_frames=True) as srv:
+        t_start = time.perf_counter()
+        nxt = 0
+        live = []
+        while nxt < len(preqs) or live:
+            now = time.perf_counter() - t_start
+            while nxt < len(preqs) and preqs[nxt].arrival_s <= now:
+                st, pos, vf = pscenes[nxt]
+                live.append(srv.submit(st, pos, vf, preqs[nxt].steps))
+                nxt += 1
+            if srv.tick() == 0 and nxt < len(preqs):
+                time.sleep(max(0.0, min(0.0005, preqs[nxt].arrival_s - now)))
+            for rid in list(live):
+                stt = srv.status(rid)
+                if stt["state"] == "done":
+                    lat.append((stt["done_s"] - stt["submit_s"]) * 1e3)
+                    srv.collect(rid)
+                    live.remove(rid)
+        wall = time.perf_counter() - t_start
+        frame_ms = srv.frame_times()
+    pwork = float(sum(r.number_of_agents * r.steps for r in preqs))
+    lat = np.array(lat)
+    return {
+        "workload": "mixed scenes: N log-uniform [4096, 1048576], G = next_pow2(ceil(sqrt N)), "
+                    "1-60 LOOP steps, plain-step distributions",
+        "requests": n_requests,
+        "spheres_total": int(sum(r.number_of_agents for r in reqs)),
+        "sphere_steps": work,
+        "batched_sphere_steps_per_s": work / batched_s,
+        "one_by_one_sphere_steps_per_s": work / one_s,
+        "batched_over_one_by_one": one_s / batched_s,
+        "batched_s": batched_s,
+        "batched_frames_device_s": batched_device_s,
+        "one_by_one_s": one_s,
+        "frames_batched": int(frames_batched),
+        "frames_one_by_one": int(frames_one),
+        "poisson": {
+            "rate_per_s": 200.0,
+            "requests": n_poisson,
+            "wall_s": wall,
+            "sphere_steps_per_s": pwork / wall,
+            "request_latency_ms_p50": float(np.percentile(lat, 50)),
+            "request_latency_ms_p99": float(np.percentile(lat, 99)),
+            "frame_ms_p50": float(np.percentile(frame_ms, 50)),
+            "frame_ms_p99": float(np.percentile(frame_ms, 99)),
+            "frames": int(len(frame_ms)),
+        },
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +194,9 @@ def main() -> int:
     ap.add_argument("--latency-steps", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=64)
+    ap.add_argument("--poisson-requests", type=int, default=200)
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -207,6 +306,8 @@ def main() -> int:
             "bytes_per_launch": phys_bytes,
         },
     }
+    if not args.no_serving:
+        line["serving"] = serving_bench(args.serving_requests, args.poisson_requests, args.seed)
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         line["cpu_baseline"] = cpu_baseline(cfg, pos, vf, args.cpu_seconds, threads)

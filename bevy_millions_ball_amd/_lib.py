@@ -48,6 +48,47 @@ class MbBuffers(C.Structure):
     ]
 
 
+class MbServerConfig(C.Structure):
+    _fields_ = [
+        ("memory_cap_bytes", C.c_uint64),
+        ("chunk_bytes", C.c_uint64),
+        ("use_graph", C.c_int32),
+        ("profile_frames", C.c_int32),
+    ]
+
+
+class MbRequestStatus(C.Structure):
+    _fields_ = [
+        ("state", C.c_int32),
+        ("number_of_agents", C.c_uint32),
+        ("steps", C.c_uint32),
+        ("frames_done", C.c_uint32),
+        ("submit_s", C.c_double),
+        ("admit_s", C.c_double),
+        ("done_s", C.c_double),
+    ]
+
+
+class MbServerStats(C.Structure):
+    _fields_ = [
+        ("queued", C.c_uint32),
+        ("running", C.c_uint32),
+        ("retiring", C.c_uint32),
+        ("done", C.c_uint32),
+        ("used_bytes", C.c_uint64),
+        ("cap_bytes", C.c_uint64),
+        ("reserved_bytes", C.c_uint64),
+        ("frames", C.c_uint64),
+        ("sphere_steps", C.c_double),
+    ]
+
+
+MB_REQUEST_QUEUED = 0
+MB_REQUEST_RUNNING = 1
+MB_REQUEST_RETIRING = 2
+MB_REQUEST_DONE = 3
+
+
 class MbError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"millions_ball error {code}: {message}")
@@ -88,6 +129,19 @@ SIGNATURES = {
         C.c_int,
         [C.POINTER(MbSettings), C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, C.c_uint32, _P],
     ),
+    "mb_server_scene_bytes": (C.c_uint64, [C.c_uint32, C.c_uint32]),
+    "mb_server_create": (C.c_int, [C.POINTER(MbServerConfig), C.POINTER(_P)]),
+    "mb_server_destroy": (C.c_int, [_P]),
+    "mb_server_submit": (
+        C.c_int,
+        [_P, C.POINTER(MbSettings), _P, _P, C.c_uint32, C.c_int, C.POINTER(C.c_uint64)],
+    ),
+    "mb_server_tick": (C.c_int, [_P, _U32P]),
+    "mb_server_drain": (C.c_int, [_P]),
+    "mb_server_status": (C.c_int, [_P, C.c_uint64, C.POINTER(MbRequestStatus)]),
+    "mb_server_collect": (C.c_int, [_P, C.c_uint64, _P, _P, _P, _P]),
+    "mb_server_stats_read": (C.c_int, [_P, C.POINTER(MbServerStats)]),
+    "mb_server_frame_times": (C.c_int, [_P, _P, C.c_uint32, _U32P]),
 }
 
 

@@ -26,6 +26,12 @@ int fail(int code, const std::string &msg)
     return code;
 }
 
+}  // namespace
+
+int mb::set_error(int code, const std::string &msg) { return fail(code, msg); }
+
+namespace {
+
 int fail_hip(hipError_t e, const char *what)
 {
     return fail(MB_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -74,7 +80,9 @@ uint32_t rust_as_u32(float f)
 float quantize_signed(float x) { return (float)rust_as_i32(x * 1000.0f) / 1000.0f; }
 float quantize_unsigned(float x) { return (float)rust_as_u32(x * 1000.0f) / 1000.0f; }
 
-mb::PhysConsts compile_consts(const mb_settings &s)
+}  // namespace
+
+mb::PhysConsts mb::compile_consts(const mb_settings &s)
 {
     mb::PhysConsts c;
     c.grids = s.number_of_grids_one_dimension;
@@ -98,6 +106,8 @@ mb::PhysConsts compile_consts(const mb_settings &s)
     c.grids_div = mb::make_fastdiv(s.number_of_grids_one_dimension);
     return c;
 }
+
+namespace {
 
 // The constraints the reference documents for its buffers and sort
 // (mod.rs:305-320, 351-362) plus what keeps every index in range here.
@@ -221,7 +231,7 @@ int mb_physics_create(const mb_settings *settings, mb_physics **out)
     mb_physics *p = new (std::nothrow) mb_physics();
     if (!p) return fail(MB_ERR_OUT_OF_MEMORY, "host allocation failed");
     p->settings = *settings;
-    p->consts = compile_consts(*settings);
+    p->consts = mb::compile_consts(*settings);
     p->n = settings->max_number_of_agents;
     if (hipGetDevice(&p->device) != hipSuccess) {
         delete p;
@@ -292,7 +302,7 @@ int mb_physics_set_settings(mb_physics *p, const mb_settings *s)
     const bool regrid = s->number_of_grids_one_dimension != p->settings.number_of_grids_one_dimension ||
                         s->grid_size != p->settings.grid_size;
     p->settings = *s;
-    p->consts = compile_consts(*s);
+    p->consts = mb::compile_consts(*s);
     if (regrid && p->time >= 0) {
         // The next frame sorts keys of the new grid (the reference would sort
         // the stale keys of the old grid once, reading stale `close` words).
@@ -524,7 +534,7 @@ int mb_physics_run(const mb_settings *s, int init, const uint32_t *perm, const u
         return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
     if (s->number_of_grids_one_dimension == 0 || s->grid_size == 0)
         return fail(MB_ERR_INVALID_ARGUMENT, "grid count and grid size must be >= 1");
-    hipError_t e = mb::launch_physics(compile_consts(*s), init != 0, perm, start, close, key_out,
+    hipError_t e = mb::launch_physics(mb::compile_consts(*s), init != 0, perm, start, close, key_out,
                                       static_cast<const float4 *>(pos_i), static_cast<const uint4 *>(vfme_i),
                                       static_cast<float4 *>(pos_o), static_cast<uint4 *>(vfme_o), n,
                                       static_cast<hipStream_t>(stream));

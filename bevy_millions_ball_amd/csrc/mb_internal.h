@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 namespace mb {
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
@@ -133,5 +135,59 @@ hipError_t launch_rekey(const PhysConsts &c, const float4 *pos, uint32_t *key_ou
                         hipStream_t stream);
 
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t count, hipStream_t stream);
+
+// ---- many independent scenes in one frame ------------------------------------
+// Every array of a scene is its own: keys, cells, rows and slots are
+// scene-local, so a scene's results do not depend on what else is resident.
+struct SceneDesc {
+    PhysConsts c;
+    uint32_t n;
+    int32_t time;          // time of the frame about to run; 0 runs INIT
+    uint32_t frames_left;  // frames still to run; 0 = idle (its blocks return at once)
+    uint32_t pad_;
+    float4 *pos[2];        // state ping-pong: frame t reads [t & 1], writes [!(t & 1)]
+    uint4 *vfme[2];
+    uint32_t *keys;        // input keys, then the sorted keys, then the next keys (in place)
+    uint32_t *perm;        // sorted slot -> index into the input state
+    uint32_t *off;         // G^2 + 1 per-cell slot offsets
+    uint32_t *rows;        // row_count [G] | row_fill [G] | row_start [G + 1]; the INIT frame zeroes the counters
+    uint64_t *pairs;       // n
+    uint32_t *scratch_idx; // n
+};
+
+// Work lists of one batched frame: (scene index, tile / row / block of the
+// scene) for every block of each kernel, scene by scene.
+struct BatchPlan {
+    SceneDesc *scenes;
+    uint32_t scene_count;
+    const uint2 *tile_map;    // tiles of kPartTile input keys
+    uint32_t tiles;
+    const uint2 *row_map;     // rows of the grid
+    uint32_t rows;
+    const uint2 *blk_plain;   // blocks of 256 sorted slots, spring off
+    uint32_t blk_plain_count;
+    const uint2 *blk_spring;  // blocks of 256 sorted slots, spring on
+    uint32_t blk_spring_count;
+    uint32_t max_grids;       // largest G among the scenes (row sort LDS)
+};
+
+// Enqueues one frame of every scene with frames_left > 0 (INIT for those at
+// time 0), then advances their time and frames_left on the device.
+hipError_t launch_batch_frame(const BatchPlan &p, hipStream_t stream);
+
+// dst[i] = src[i] for i < words, by a kernel (src may be mapped host memory).
+hipError_t launch_copy_words(uint32_t *dst, const uint32_t *src, uint32_t words,
+                             hipStream_t stream);
+
+}  // namespace mb
+
+struct mb_settings;
+
+namespace mb {
+
+// Host helpers shared by the C ABI files.
+PhysConsts compile_consts(const mb_settings &s);
+// Stores `msg` as the calling thread's last error and returns `code`.
+int set_error(int code, const std::string &msg);
 
 }  // namespace mb

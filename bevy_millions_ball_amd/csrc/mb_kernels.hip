@@ -1,19 +1,22 @@
 // mb_kernels.hip -- CDNA4 (gfx950) kernels of the physics step.
 //
-// Stages (reference: src/physics/mod.rs:880-909):
-//   1. stable LSD radix sort of u32 cell keys with u32 indices
-//      (replaces bevy_radix_sort::run, mod.rs:880-890);
-//   2. spatial hashing: per-cell [start, close] of the sorted keys
-//      (spatial_hashing.wgsl:29-70);
-//   3. physics INIT / LOOP: collide + Verlet integrate + next cell key
-//      (physics.wgsl:57-97).
+// The reference's frame (src/physics/mod.rs:880-909) is bevy_radix_sort ->
+// spatial_hashing.wgsl -> physics.wgsl.  Two forms are built here:
+//   - the stage kernels behind the stage entry points: a stable LSD radix
+//     sort of u32 keys with u32 indices (8-bit digits), spatial hashing
+//     (per-cell start / close, spatial_hashing.wgsl:29-70) and the physics
+//     INIT / LOOP pass on start / close (physics.wgsl:57-97);
+//   - the frame path: row count -> row partition -> row sort (which also
+//     writes per-cell offsets) -> collide + integrate, for one scene or for
+//     many scenes in one set of launches.
 //
 // Floating point follows physics.wgsl operation by operation: the library is
 // built with -ffp-contract=off, sqrtf and '/' are the correctly rounded HIP
 // defaults, min/max/clamp are compare-and-select, f32 -> f16 rounds to
-// nearest even.  Every stage is integer or elementwise work bound by HBM
-// bandwidth; nothing is a dense contraction, so there is no MFMA here.
+// nearest even.  Nothing is a dense contraction, so there is no MFMA here.
 #include "mb_internal.h"
+
+#include <algorithm>
 
 namespace mb {
 
@@ -931,14 +934,14 @@ __device__ __forceinline__ void block_row_span(uint32_t lo, uint32_t hi, uint32_
 // Row histogram of the frame's input keys: each block counts its tile in LDS
 // and adds one atomic per row it touches (nearly sorted keys touch few rows,
 // so every row counter sees about as many atomics as blocks overlap it).
-__global__ __launch_bounds__(256) void row_count_kernel(const uint32_t *__restrict__ keys,
-                                                        uint32_t n, uint32_t grids, FastDiv gdiv,
-                                                        uint32_t *__restrict__ row_count)
+__device__ __forceinline__ void row_count_body(const uint32_t *__restrict__ keys, uint32_t n,
+                                               uint32_t grids, FastDiv gdiv,
+                                               uint32_t *__restrict__ row_count, uint32_t tile)
 {
     __shared__ uint32_t s_hist[kMaxRows];
     __shared__ uint32_t s_lo, s_hi;
     const uint32_t tid = threadIdx.x;
-    const uint32_t base = blockIdx.x * kPartTile;
+    const uint32_t base = tile * kPartTile;
     uint32_t row[kPartItems];
     uint32_t lo = 0xFFFFFFFFu, hi = 0u;
 #pragma unroll
@@ -965,10 +968,17 @@ __global__ __launch_bounds__(256) void row_count_kernel(const uint32_t *__restri
     }
 }
 
-__global__ __launch_bounds__(256) void row_partition_kernel(
+__global__ __launch_bounds__(256) void row_count_kernel(const uint32_t *__restrict__ keys,
+                                                        uint32_t n, uint32_t grids, FastDiv gdiv,
+                                                        uint32_t *__restrict__ row_count)
+{
+    row_count_body(keys, n, grids, gdiv, row_count, blockIdx.x);
+}
+
+__device__ __forceinline__ void row_partition_body(
     const uint32_t *__restrict__ keys, uint32_t n, uint32_t grids, FastDiv gdiv,
     const uint32_t *__restrict__ row_count, uint32_t *__restrict__ row_fill,
-    uint32_t *__restrict__ row_start_out, uint64_t *__restrict__ pairs)
+    uint32_t *__restrict__ row_start_out, uint64_t *__restrict__ pairs, uint32_t tile)
 {
     __shared__ __align__(16) uint32_t s_start[kMaxRows + 4];
     __shared__ uint32_t s_mine[kMaxRows];  // this block's count, then its base, per row
@@ -981,10 +991,10 @@ __global__ __launch_bounds__(256) void row_partition_kernel(
     const uint32_t total = block256_scan_lds(s_start, rows, s_scratch);
     if (tid == 0) s_start[rows] = total;
     __syncthreads();
-    if (blockIdx.x == 0)
+    if (tile == 0)
         for (uint32_t r = tid; r <= rows; r += 256u) row_start_out[r] = s_start[r];
 
-    const uint32_t base = blockIdx.x * kPartTile;
+    const uint32_t base = tile * kPartTile;
     uint32_t key[kPartItems];
     uint32_t slot[kPartItems];
     uint32_t lo = 0xFFFFFFFFu, hi = 0u;
@@ -1028,20 +1038,28 @@ __global__ __launch_bounds__(256) void row_partition_kernel(
     }
 }
 
+__global__ __launch_bounds__(256) void row_partition_kernel(
+    const uint32_t *__restrict__ keys, uint32_t n, uint32_t grids, FastDiv gdiv,
+    const uint32_t *__restrict__ row_count, uint32_t *__restrict__ row_fill,
+    uint32_t *__restrict__ row_start_out, uint64_t *__restrict__ pairs)
+{
+    row_partition_body(keys, n, grids, gdiv, row_count, row_fill, row_start_out, pairs, blockIdx.x);
+}
+
 // One block per row.  Dynamic LDS: `grids` cell counters (padded to a
 // multiple of 1024 words for the vector scan) followed by kRowCap indices.
-__global__ __launch_bounds__(256) void row_sort_kernel(
+__device__ __forceinline__ void row_sort_body(
     const uint64_t *__restrict__ pairs, uint32_t *__restrict__ scratch_idx,
     const uint32_t *__restrict__ row_start, uint32_t *__restrict__ row_count,
     uint32_t *__restrict__ row_fill, uint32_t grids, uint32_t n,
-    uint32_t *__restrict__ perm, uint32_t *__restrict__ off, uint32_t *__restrict__ sorted_keys)
+    uint32_t *__restrict__ perm, uint32_t *__restrict__ off, uint32_t *__restrict__ sorted_keys,
+    uint32_t r)
 {
     extern __shared__ __align__(16) uint32_t s_dyn[];
     uint32_t *s_cell = s_dyn;                              // grids words (padded)
     uint32_t *s_idx = s_dyn + ((grids + 1023u) & ~1023u);  // kRowCap words
     __shared__ uint32_t s_scratch[4];
     const uint32_t tid = threadIdx.x;
-    const uint32_t r = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t base = row_start[r];
     const uint32_t m = row_start[r + 1u] - base;
     for (uint32_t c = tid; c < grids; c += 256u) s_cell[c] = 0u;
@@ -1130,6 +1148,16 @@ __global__ __launch_bounds__(256) void row_sort_kernel(
         row_fill[r] = 0u;
         if (r == grids - 1u) off[grids * grids] = n;
     }
+}
+
+__global__ __launch_bounds__(256) void row_sort_kernel(
+    const uint64_t *__restrict__ pairs, uint32_t *__restrict__ scratch_idx,
+    const uint32_t *__restrict__ row_start, uint32_t *__restrict__ row_count,
+    uint32_t *__restrict__ row_fill, uint32_t grids, uint32_t n,
+    uint32_t *__restrict__ perm, uint32_t *__restrict__ off, uint32_t *__restrict__ sorted_keys)
+{
+    row_sort_body(pairs, scratch_idx, row_start, row_count, row_fill, grids, n, perm, off,
+                  sorted_keys, xcd_block(blockIdx.x, gridDim.x));
 }
 
 // solve_sphere_constraint behind a cheap conservative filter:
@@ -1272,17 +1300,13 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
 // last cells come from the sorted keys, so the window bounds are uniform
 // loads issued at launch and one barrier separates staging from the walk.
 template <bool SPRING>
-__global__ __launch_bounds__(256) void physics_step_kernel(
-    PhysConsts c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
+__device__ __forceinline__ void physics_step_body(
+    const PhysConsts &c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
     uint32_t *__restrict__ keys, const float4 *__restrict__ pos_i,
     const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
-    uint32_t n)
+    uint32_t n, uint32_t blk, float4 *__restrict__ s_pos, uint2 *__restrict__ s_vm)
 {
-    __shared__ float4 s_pos[3 * kStageCap];  // the three windows back to back
-    __shared__ uint2 s_vm[3 * kStageCap];    // their velocity and mass halves
-
     const uint32_t tid = threadIdx.x;
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t s0 = blk * 256u;
     const uint32_t last = min(n - s0, 256u) - 1u;
     const uint32_t s = s0 + min(tid, last);  // idle threads mirror the last slot
@@ -1353,15 +1377,27 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     (void)finish_agent<SPRING>(c, a, s, pos_o, vfme_o, keys);
 }
 
-// INIT frame of the frame path: the sort of the all-zero keys is the
-// identity, so perm[s] = s and every sphere sits in cell 0's range.
-__global__ __launch_bounds__(256) void physics_init_step_kernel(
-    PhysConsts c, uint32_t *__restrict__ perm, uint32_t *__restrict__ key_out,
-    const float4 *__restrict__ pos_i,
+template <bool SPRING>
+__global__ __launch_bounds__(256) void physics_step_kernel(
+    PhysConsts c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
+    uint32_t *__restrict__ keys, const float4 *__restrict__ pos_i,
     const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
     uint32_t n)
 {
-    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    __shared__ float4 s_pos[3 * kStageCap];  // the three windows back to back
+    __shared__ uint2 s_vm[3 * kStageCap];    // their velocity and mass halves
+    physics_step_body<SPRING>(c, perm, off, keys, pos_i, vfme_i, pos_o, vfme_o, n,
+                              xcd_block(blockIdx.x, gridDim.x), s_pos, s_vm);
+}
+
+// INIT frame of the frame path: the sort of the all-zero keys is the
+// identity, so perm[s] = s and every sphere sits in cell 0's range.
+__device__ __forceinline__ void physics_init_body(
+    const PhysConsts &c, uint32_t *__restrict__ perm, uint32_t *__restrict__ key_out,
+    const float4 *__restrict__ pos_i,
+    const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
+    uint32_t n, uint32_t s)
+{
     if (s >= n) return;
     perm[s] = s;
     const float4 p = pos_i[s];
@@ -1372,6 +1408,16 @@ __global__ __launch_bounds__(256) void physics_init_step_kernel(
     vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
                            pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
     key_out[s] = cz * c.grids + cx;
+}
+
+__global__ __launch_bounds__(256) void physics_init_step_kernel(
+    PhysConsts c, uint32_t *__restrict__ perm, uint32_t *__restrict__ key_out,
+    const float4 *__restrict__ pos_i,
+    const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
+    uint32_t n)
+{
+    physics_init_body(c, perm, key_out, pos_i, vfme_i, pos_o, vfme_o, n,
+                      blockIdx.x * 256u + threadIdx.x);
 }
 
 // Offsets of the INIT frame: cell 0 holds every sphere.
@@ -1433,6 +1479,144 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, bool init, u
                            b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, n);
     }
     if ((e = mark(3)) != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+// ---- many scenes in one frame ------------------------------------------------
+// Each block looks up (scene, index within the scene) in its kernel's work
+// list and runs the single-scene body on that scene's arrays, so a scene's
+// results are the same alone and in any mix.  A scene at time 0 runs the
+// INIT frame: the sort kernels skip it (its keys are not read), the row sort
+// blocks write its INIT offsets and the collide blocks copy its state.
+
+// Pointers loaded from a SceneDesc are generic; the round trip through the
+// global address space lets the compiler use global (not flat) accesses.
+template <typename T>
+__device__ __forceinline__ T *gptr(T *p)
+{
+    typedef __attribute__((address_space(1))) T GlobalT;
+    GlobalT *g = (GlobalT *)(uintptr_t)p;
+    return (T *)g;
+}
+
+__global__ __launch_bounds__(256) void batch_row_count_kernel(const SceneDesc *__restrict__ scenes,
+                                                              const uint2 *__restrict__ map)
+{
+    const uint2 m = map[blockIdx.x];
+    const SceneDesc &S = scenes[m.x];
+    if (S.frames_left == 0u || S.time == 0) return;
+    row_count_body(gptr(S.keys), S.n, S.c.grids, S.c.grids_div, gptr(S.rows), m.y);
+}
+
+__global__ __launch_bounds__(256) void batch_row_partition_kernel(
+    const SceneDesc *__restrict__ scenes, const uint2 *__restrict__ map)
+{
+    const uint2 m = map[blockIdx.x];
+    const SceneDesc &S = scenes[m.x];
+    if (S.frames_left == 0u || S.time == 0) return;
+    const uint32_t G = S.c.grids;
+    uint32_t *rows = gptr(S.rows);
+    row_partition_body(gptr(S.keys), S.n, G, S.c.grids_div, rows, rows + G, rows + 2u * G,
+                       gptr(S.pairs), m.y);
+}
+
+__global__ __launch_bounds__(256) void batch_row_sort_kernel(const SceneDesc *__restrict__ scenes,
+                                                             const uint2 *__restrict__ map)
+{
+    const uint2 m = map[xcd_block(blockIdx.x, gridDim.x)];
+    const SceneDesc &S = scenes[m.x];
+    if (S.frames_left == 0u) return;
+    const uint32_t G = S.c.grids;
+    uint32_t *off = gptr(S.off);
+    uint32_t *rows = gptr(S.rows);
+    if (S.time == 0) {
+        // INIT offsets (cell 0 holds every sphere), and the row counters the
+        // first LOOP frame expects to find zero
+        for (uint32_t c = threadIdx.x; c < G; c += 256u)
+            off[m.y * G + c] = (m.y == 0u && c == 0u) ? 0u : S.n;
+        if (threadIdx.x == 0) {
+            if (m.y == G - 1u) off[G * G] = S.n;
+            rows[m.y] = 0u;      // row_count
+            rows[G + m.y] = 0u;  // row_fill
+        }
+        return;
+    }
+    row_sort_body(gptr(S.pairs), gptr(S.scratch_idx), rows + 2u * G, rows, rows + G, G, S.n,
+                  gptr(S.perm), off, gptr(S.keys), m.y);
+}
+
+template <bool SPRING>
+__global__ __launch_bounds__(256) void batch_physics_step_kernel(
+    const SceneDesc *__restrict__ scenes, const uint2 *__restrict__ map)
+{
+    __shared__ float4 s_pos[3 * kStageCap];
+    __shared__ uint2 s_vm[3 * kStageCap];
+    const uint2 m = map[xcd_block(blockIdx.x, gridDim.x)];
+    const SceneDesc &S = scenes[m.x];
+    if (S.frames_left == 0u) return;
+    if (S.time == 0) {
+        physics_init_body(S.c, gptr(S.perm), gptr(S.keys), gptr(S.pos[0]), gptr(S.vfme[0]),
+                          gptr(S.pos[1]), gptr(S.vfme[1]), S.n, m.y * 256u + threadIdx.x);
+        return;
+    }
+    const uint32_t p = (uint32_t)S.time & 1u;
+    physics_step_body<SPRING>(S.c, gptr(S.perm), gptr(S.off), gptr(S.keys), gptr(S.pos[p]),
+                              gptr(S.vfme[p]), gptr(S.pos[p ^ 1u]), gptr(S.vfme[p ^ 1u]), S.n,
+                              m.y, s_pos, s_vm);
+}
+
+__global__ __launch_bounds__(256) void batch_advance_kernel(SceneDesc *__restrict__ scenes,
+                                                            uint32_t count)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t left = scenes[i].frames_left;
+    if (left == 0u) return;
+    scenes[i].time += 1;
+    scenes[i].frames_left = left - 1u;
+}
+
+// Plan tables from mapped, coherent host memory into device memory: a kernel
+// on the frames' stream, so the copy never queues behind bulk DMA transfers.
+__global__ __launch_bounds__(256) void copy_words_kernel(uint32_t *__restrict__ dst,
+                                                         const uint32_t *__restrict__ src,
+                                                         uint32_t words)
+{
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < words; i += gridDim.x * 256u)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy_words(uint32_t *dst, const uint32_t *src, uint32_t words,
+                             hipStream_t stream)
+{
+    if (words == 0) return hipSuccess;
+    const uint32_t blocks = std::min<uint32_t>((words + 255u) / 256u, 1024u);
+    hipLaunchKernelGGL(copy_words_kernel, dim3(blocks), dim3(256), 0, stream, dst, src, words);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_frame(const BatchPlan &p, hipStream_t stream)
+{
+    if (p.scene_count == 0) return hipSuccess;
+    if (p.tiles) {
+        hipLaunchKernelGGL(batch_row_count_kernel, dim3(p.tiles), dim3(256), 0, stream, p.scenes,
+                           p.tile_map);
+        hipLaunchKernelGGL(batch_row_partition_kernel, dim3(p.tiles), dim3(256), 0, stream,
+                           p.scenes, p.tile_map);
+    }
+    if (p.rows) {
+        const size_t sort_lds = ((size_t)((p.max_grids + 1023u) & ~1023u) + kRowCap) * 4u;
+        hipLaunchKernelGGL(batch_row_sort_kernel, dim3(p.rows), dim3(256), sort_lds, stream,
+                           p.scenes, p.row_map);
+    }
+    if (p.blk_plain_count)
+        hipLaunchKernelGGL(batch_physics_step_kernel<false>, dim3(p.blk_plain_count), dim3(256), 0,
+                           stream, p.scenes, p.blk_plain);
+    if (p.blk_spring_count)
+        hipLaunchKernelGGL(batch_physics_step_kernel<true>, dim3(p.blk_spring_count), dim3(256), 0,
+                           stream, p.scenes, p.blk_spring);
+    hipLaunchKernelGGL(batch_advance_kernel, dim3((p.scene_count + 255u) / 256u), dim3(256), 0,
+                       stream, p.scenes, p.scene_count);
     return hipGetLastError();
 }
 

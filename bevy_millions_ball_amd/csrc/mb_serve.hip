@@ -1,0 +1,817 @@
+// mb_serve.hip -- the serving half of include/millions_ball.h: many
+// independent scenes resident in device memory, stepped together.
+//
+// Design:
+//   - Device memory comes from a chunked arena (first fit, coalescing free
+//     lists); each scene holds one region with all of its arrays.  Admission
+//     is first come, first served against a byte cap on the regions.
+//   - Each tick runs one batched frame (mb::launch_batch_frame): five
+//     launches cover every resident scene, whatever their sizes and grids.
+//     Per-scene time and frames_left live in the device scene table and are
+//     advanced by the frame's last kernel, so an unchanged composition can
+//     be replayed as one captured hipGraph.  The host keeps a mirror of both
+//     counters and rewrites the table only when the composition changes.
+//   - Uploads and retirement downloads run on a second stream, so they
+//     overlap other scenes' frames.  An admitted scene joins the frames once
+//     its upload has completed; a retired scene's download waits for its
+//     last frame (an event), and its region returns to the arena only after
+//     that download has completed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/millions_ball.h"
+#include "mb_internal.h"
+
+namespace {
+
+constexpr uint64_t kAlign = 256;
+constexpr uint32_t kDepth = 2;  // frames in flight before a tick waits
+
+uint64_t align_up(uint64_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+double now_s()
+{
+    using clock = std::chrono::steady_clock;
+    return std::chrono::duration<double>(clock::now().time_since_epoch()).count();
+}
+
+int fail(int code, const std::string &msg) { return mb::set_error(code, msg); }
+
+int fail_hip(hipError_t e, const char *what)
+{
+    return fail(MB_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define MB_HIP(call)                                      \
+    do {                                                  \
+        hipError_t _e = (call);                           \
+        if (_e != hipSuccess) return fail_hip(_e, #call); \
+    } while (0)
+
+bool finite_half(uint32_t h16) { return ((h16 >> 10) & 0x1Fu) != 0x1Fu; }
+
+// Byte offsets of one scene's arrays inside its region.
+struct SceneLayout {
+    uint64_t pos[2], vfme[2], keys, perm, scratch, pairs, off, rows, bytes;
+};
+
+SceneLayout scene_layout(uint64_t n, uint64_t g)
+{
+    SceneLayout l;
+    uint64_t at = 0;
+    auto take = [&](uint64_t bytes) {
+        const uint64_t o = at;
+        at = align_up(at + bytes);
+        return o;
+    };
+    l.pos[0] = take(n * 16);
+    l.pos[1] = take(n * 16);
+    l.vfme[0] = take(n * 16);
+    l.vfme[1] = take(n * 16);
+    l.keys = take(n * 4);
+    l.perm = take(n * 4);
+    l.scratch = take(n * 4);
+    l.pairs = take(n * 8);
+    l.off = take((g * g + 1) * 4);
+    l.rows = take((3 * g + 1) * 4);
+    l.bytes = at;
+    return l;
+}
+
+// Pinned host buffers, reused by power-of-two size class.
+class PinnedPool {
+public:
+    ~PinnedPool()
+    {
+        for (auto &kv : free_)
+            for (void *p : kv.second) (void)hipHostFree(p);
+    }
+    void *get(size_t bytes, size_t &cls)
+    {
+        cls = 65536;
+        while (cls < bytes) cls <<= 1;
+        auto &v = free_[cls];
+        if (!v.empty()) {
+            void *p = v.back();
+            v.pop_back();
+            return p;
+        }
+        void *p = nullptr;
+        if (hipHostMalloc(&p, cls, hipHostMallocDefault) != hipSuccess) return nullptr;
+        return p;
+    }
+    void put(void *p, size_t cls)
+    {
+        if (p) free_[cls].push_back(p);
+    }
+
+private:
+    std::map<size_t, std::vector<void *>> free_;
+};
+
+// Device arena: chunks from hipMalloc, first-fit regions, coalescing frees.
+class Arena {
+public:
+    explicit Arena(uint64_t chunk) : chunk_(chunk ? chunk : (1ull << 30)) {}
+    ~Arena()
+    {
+        for (auto &c : chunks_) (void)hipFree(c.base);
+    }
+    char *alloc(uint64_t bytes)
+    {
+        for (auto &c : chunks_) {
+            if (char *p = take(c, bytes)) return p;
+        }
+        Chunk c;
+        c.bytes = std::max(chunk_, align_up(bytes));
+        if (hipMalloc(&c.base, c.bytes) != hipSuccess) return nullptr;
+        c.free[0] = c.bytes;
+        chunks_.push_back(c);
+        reserved_ += c.bytes;
+        return take(chunks_.back(), bytes);
+    }
+    void release(char *p, uint64_t bytes)
+    {
+        for (auto &c : chunks_) {
+            if (p < c.base || p >= c.base + c.bytes) continue;
+            uint64_t off = (uint64_t)(p - c.base), len = align_up(bytes);
+            auto next = c.free.lower_bound(off);
+            if (next != c.free.end() && off + len == next->first) {
+                len += next->second;
+                next = c.free.erase(next);
+            }
+            if (next != c.free.begin()) {
+                auto prev = std::prev(next);
+                if (prev->first + prev->second == off) {
+                    prev->second += len;
+                    return;
+                }
+            }
+            c.free[off] = len;
+            return;
+        }
+    }
+    uint64_t reserved() const { return reserved_; }
+
+private:
+    struct Chunk {
+        char *base = nullptr;
+        uint64_t bytes = 0;
+        std::map<uint64_t, uint64_t> free;  // offset -> length
+    };
+    static char *take(Chunk &c, uint64_t bytes)
+    {
+        const uint64_t need = align_up(bytes);
+        for (auto it = c.free.begin(); it != c.free.end(); ++it) {
+            if (it->second < need) continue;
+            const uint64_t off = it->first, len = it->second;
+            c.free.erase(it);
+            if (len > need) c.free[off + need] = len - need;
+            return c.base + off;
+        }
+        return nullptr;
+    }
+    uint64_t chunk_;
+    uint64_t reserved_ = 0;
+    std::vector<Chunk> chunks_;
+};
+
+struct Request {
+    uint64_t id = 0;
+    mb_settings settings{};
+    mb::PhysConsts consts{};
+    uint32_t n = 0, g = 0, steps = 0;
+    bool keep_debug = false;
+    int state = MB_REQUEST_QUEUED;
+    SceneLayout layout{};
+    char *region = nullptr;
+    int32_t time_next = 0;      // host mirror of the scene table
+    uint32_t frames_left = 0;
+    uint32_t frames_done = 0;
+    // pinned host buffer: the initial pos (16 n) | vfme (16 n) until the
+    // upload lands, then the results pos | vfme [| perm (4 n) | keys (4 n)]
+    void *io = nullptr;
+    size_t io_cls = 0;
+    hipEvent_t ev_upload = nullptr, ev_frame = nullptr, ev_done = nullptr;
+    double submit_s = 0, admit_s = 0, done_s = 0;
+
+    mb::SceneDesc desc() const
+    {
+        mb::SceneDesc d{};
+        d.c = consts;
+        d.n = n;
+        d.time = time_next;
+        d.frames_left = frames_left;
+        for (int i = 0; i < 2; ++i) {
+            d.pos[i] = reinterpret_cast<float4 *>(region + layout.pos[i]);
+            d.vfme[i] = reinterpret_cast<uint4 *>(region + layout.vfme[i]);
+        }
+        d.keys = reinterpret_cast<uint32_t *>(region + layout.keys);
+        d.perm = reinterpret_cast<uint32_t *>(region + layout.perm);
+        d.off = reinterpret_cast<uint32_t *>(region + layout.off);
+        d.rows = reinterpret_cast<uint32_t *>(region + layout.rows);
+        d.pairs = reinterpret_cast<uint64_t *>(region + layout.pairs);
+        d.scratch_idx = reinterpret_cast<uint32_t *>(region + layout.scratch);
+        return d;
+    }
+    size_t out_bytes() const { return (size_t)n * (32 + (keep_debug ? 8 : 0)); }
+};
+
+int validate_scene(const mb_settings &s)
+{
+    const uint64_t n = s.max_number_of_agents;
+    const uint64_t g = s.number_of_grids_one_dimension;
+    if (n == 0 || n >= (1ull << 30))
+        return fail(MB_ERR_INVALID_ARGUMENT, "max_number_of_agents must be in [1, 2^30)");
+    if (g == 0) return fail(MB_ERR_INVALID_ARGUMENT, "number_of_grids_one_dimension must be >= 1");
+    if (s.grid_size == 0) return fail(MB_ERR_INVALID_ARGUMENT, "grid_size must be >= 1");
+    if (g * g > (1ull << 24))
+        return fail(MB_ERR_INVALID_ARGUMENT,
+                    "number_of_grids_one_dimension^2 exceeds 2^24 (the 24-bit cell key)");
+    if (g * (uint64_t)s.grid_size > 0xFFFFFFFFull)
+        return fail(MB_ERR_INVALID_ARGUMENT, "number_of_grids_one_dimension * grid_size overflows u32");
+    return MB_OK;
+}
+
+}  // namespace
+
+struct mb_server {
+    mb_server_config config{};
+    int device = 0;
+    hipStream_t compute = nullptr, copy = nullptr;
+    std::unique_ptr<Arena> arena;
+    PinnedPool pinned;
+    uint64_t used = 0;
+    uint64_t next_id = 1;
+    std::deque<Request *> queue;
+    std::deque<Request *> uploading;  // admitted, state still on its way to the device
+    std::vector<Request *> running;   // plan order
+    std::vector<Request *> retiring;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
+    bool dirty = true;
+
+    // plan: scene table + work lists in one device buffer, staged in mapped,
+    // coherent host memory (double buffered) and copied by a kernel
+    char *d_plan = nullptr;
+    size_t plan_cap = 0;
+    // buffers outgrown while frames may still read them: hipFree and
+    // hipHostFree wait for the whole device, so they are released only when
+    // the server is destroyed
+    std::vector<void *> old_device, old_host;
+    void *stage[2] = {nullptr, nullptr};
+    size_t stage_bytes[2] = {0, 0};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    int stage_k = 0;
+    mb::BatchPlan plan{};
+
+    hipGraphExec_t exec = nullptr;
+    std::vector<std::pair<hipGraphExec_t, hipEvent_t>> graveyard;
+
+    // frames in flight, and per-frame timing
+    std::deque<hipEvent_t> inflight;
+    std::vector<hipEvent_t> spare_events;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> timed;
+    std::vector<double> frame_ms;
+
+    uint64_t frames = 0;
+    double sphere_steps = 0;
+
+    ~mb_server()
+    {
+        if (compute) (void)hipStreamSynchronize(compute);
+        if (copy) (void)hipStreamSynchronize(copy);
+        for (auto &kv : requests) {
+            Request *r = kv.second.get();
+            if (r->ev_upload) (void)hipEventDestroy(r->ev_upload);
+            if (r->ev_frame) (void)hipEventDestroy(r->ev_frame);
+            if (r->ev_done) (void)hipEventDestroy(r->ev_done);
+            pinned.put(r->io, r->io_cls);
+        }
+        if (exec) (void)hipGraphExecDestroy(exec);
+        for (auto &g : graveyard) {
+            (void)hipGraphExecDestroy(g.first);
+            (void)hipEventDestroy(g.second);
+        }
+        for (hipEvent_t e : inflight) (void)hipEventDestroy(e);
+        for (hipEvent_t e : spare_events) (void)hipEventDestroy(e);
+        for (auto &t : timed) {
+            (void)hipEventDestroy(t.first);
+            (void)hipEventDestroy(t.second);
+        }
+        for (int k = 0; k < 2; ++k) {
+            if (stage[k]) (void)hipHostFree(stage[k]);
+            if (stage_ev[k]) (void)hipEventDestroy(stage_ev[k]);
+        }
+        if (d_plan) (void)hipFree(d_plan);
+        for (void *p : old_device) (void)hipFree(p);
+        for (void *p : old_host) (void)hipHostFree(p);
+        arena.reset();
+        if (compute) (void)hipStreamDestroy(compute);
+        if (copy) (void)hipStreamDestroy(copy);
+    }
+
+    hipEvent_t take_event()
+    {
+        if (!spare_events.empty()) {
+            hipEvent_t e = spare_events.back();
+            spare_events.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+        return e;
+    }
+};
+
+namespace {
+
+// Frees the region of a request whose retirement copy has completed.
+void finish_retirement(mb_server *s, Request *r)
+{
+    s->arena->release(r->region, r->layout.bytes);
+    s->used -= r->layout.bytes;
+    r->region = nullptr;
+    r->state = MB_REQUEST_DONE;
+    r->done_s = now_s();
+    // both events have been waited on by work that is now complete
+    for (hipEvent_t *e : {&r->ev_upload, &r->ev_frame}) {
+        if (*e) s->spare_events.push_back(*e);
+        *e = nullptr;
+    }
+}
+
+int poll_retiring(mb_server *s)
+{
+    for (size_t i = 0; i < s->retiring.size();) {
+        Request *r = s->retiring[i];
+        const hipError_t q = hipEventQuery(r->ev_done);
+        if (q == hipErrorNotReady) {
+            ++i;
+            continue;
+        }
+        if (q != hipSuccess) return fail_hip(q, "retirement copy");
+        finish_retirement(s, r);
+        s->retiring.erase(s->retiring.begin() + (long)i);
+    }
+    // graphs whose last replay has finished
+    for (size_t i = 0; i < s->graveyard.size();) {
+        if (hipEventQuery(s->graveyard[i].second) == hipSuccess) {
+            (void)hipGraphExecDestroy(s->graveyard[i].first);
+            (void)hipEventDestroy(s->graveyard[i].second);
+            s->graveyard.erase(s->graveyard.begin() + (long)i);
+        } else {
+            ++i;
+        }
+    }
+    return MB_OK;
+}
+
+int admit(mb_server *s)
+{
+    while (!s->queue.empty()) {
+        Request *r = s->queue.front();
+        if (s->used + r->layout.bytes > s->config.memory_cap_bytes) break;
+        char *region = s->arena->alloc(r->layout.bytes);
+        if (!region) break;  // the device is full: wait for retirements
+        r->region = region;
+        s->used += r->layout.bytes;
+        s->queue.pop_front();
+        const mb::SceneDesc d = r->desc();
+        const size_t n = r->n;
+        const char *in = static_cast<const char *>(r->io);
+        MB_HIP(hipMemcpyAsync(d.pos[0], in, n * 16, hipMemcpyHostToDevice, s->copy));
+        MB_HIP(hipMemcpyAsync(d.vfme[0], in + n * 16, n * 16, hipMemcpyHostToDevice, s->copy));
+        r->ev_upload = s->take_event();
+        if (!r->ev_upload) return fail(MB_ERR_HIP, "event creation failed");
+        MB_HIP(hipEventRecord(r->ev_upload, s->copy));
+        r->state = MB_REQUEST_RUNNING;
+        r->admit_s = now_s();
+        r->time_next = 0;
+        r->frames_left = r->steps + 1;
+        s->uploading.push_back(r);
+    }
+    return MB_OK;
+}
+
+// Scenes whose upload has completed join the next frame (in admission
+// order), so a frame never waits for another scene's copy.
+int join_uploaded(mb_server *s)
+{
+    while (!s->uploading.empty()) {
+        Request *r = s->uploading.front();
+        const hipError_t q = hipEventQuery(r->ev_upload);
+        if (q == hipErrorNotReady) break;
+        if (q != hipSuccess) return fail_hip(q, "upload");
+        s->uploading.pop_front();
+        s->running.push_back(r);
+        s->dirty = true;
+    }
+    return MB_OK;
+}
+
+// Rebuilds the scene table and work lists of the running scenes and enqueues
+// their upload on the compute stream.
+int upload_plan(mb_server *s)
+{
+    const uint32_t count = (uint32_t)s->running.size();
+    std::vector<mb::SceneDesc> descs;
+    std::vector<uint2> tiles, rows, plain, spring;
+    descs.reserve(count);
+    uint32_t max_g = 1;
+    for (uint32_t i = 0; i < count; ++i) {
+        const Request *r = s->running[i];
+        descs.push_back(r->desc());
+        max_g = std::max(max_g, r->g);
+        for (uint32_t t = 0; t < (r->n + mb::kPartTile - 1) / mb::kPartTile; ++t)
+            tiles.push_back(make_uint2(i, t));
+        for (uint32_t g = 0; g < r->g; ++g) rows.push_back(make_uint2(i, g));
+        auto &blk = r->consts.enable_spring ? spring : plain;
+        for (uint32_t b = 0; b < (r->n + 255u) / 256u; ++b) blk.push_back(make_uint2(i, b));
+    }
+    const size_t desc_bytes = align_up(descs.size() * sizeof(mb::SceneDesc));
+    const size_t map_count = tiles.size() + rows.size() + plain.size() + spring.size();
+    const size_t need = desc_bytes + map_count * sizeof(uint2);
+    if (s->plan_cap < need) {
+        // grow geometrically; frames in flight may still read the old buffer
+        if (s->d_plan) s->old_device.push_back(s->d_plan);
+        s->d_plan = nullptr;
+        s->plan_cap = std::max(need, 2 * s->plan_cap);
+        MB_HIP(hipMalloc(&s->d_plan, s->plan_cap));
+    }
+    // stage (double buffered: wait for the copy that last read this buffer)
+    const int k = s->stage_k;
+    s->stage_k ^= 1;
+    if (s->stage_ev[k]) MB_HIP(hipEventSynchronize(s->stage_ev[k]));
+    else MB_HIP(hipEventCreateWithFlags(&s->stage_ev[k], hipEventDisableTiming));
+    if (s->stage_bytes[k] < need) {
+        if (s->stage[k]) s->old_host.push_back(s->stage[k]);
+        s->stage[k] = nullptr;
+        s->stage_bytes[k] = std::max(need, 2 * s->stage_bytes[k]);
+        MB_HIP(hipHostMalloc(&s->stage[k], s->stage_bytes[k],
+                             hipHostMallocMapped | hipHostMallocCoherent));
+    }
+    char *st = static_cast<char *>(s->stage[k]);
+    std::memcpy(st, descs.data(), descs.size() * sizeof(mb::SceneDesc));
+    uint2 *m = reinterpret_cast<uint2 *>(st + desc_bytes);
+    size_t at = 0;
+    for (auto *v : {&tiles, &rows, &plain, &spring}) {
+        std::memcpy(m + at, v->data(), v->size() * sizeof(uint2));
+        at += v->size();
+    }
+    void *src = nullptr;
+    MB_HIP(hipHostGetDevicePointer(&src, st, 0));
+    hipError_t ce = mb::launch_copy_words(reinterpret_cast<uint32_t *>(s->d_plan),
+                                          static_cast<const uint32_t *>(src),
+                                          (uint32_t)(need / 4), s->compute);
+    if (ce != hipSuccess) return fail_hip(ce, "plan copy");
+    MB_HIP(hipEventRecord(s->stage_ev[k], s->compute));
+
+    mb::BatchPlan &p = s->plan;
+    p.scenes = reinterpret_cast<mb::SceneDesc *>(s->d_plan);
+    p.scene_count = count;
+    p.tile_map = reinterpret_cast<const uint2 *>(s->d_plan + desc_bytes);
+    p.tiles = (uint32_t)tiles.size();
+    p.row_map = p.tile_map + tiles.size();
+    p.rows = (uint32_t)rows.size();
+    p.blk_plain = p.row_map + rows.size();
+    p.blk_plain_count = (uint32_t)plain.size();
+    p.blk_spring = p.blk_plain + plain.size();
+    p.blk_spring_count = (uint32_t)spring.size();
+    p.max_grids = max_g;
+
+    if (s->config.use_graph) {
+        if (s->exec) {
+            hipEvent_t e = nullptr;
+            MB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            MB_HIP(hipEventRecord(e, s->compute));
+            s->graveyard.emplace_back(s->exec, e);
+            s->exec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        MB_HIP(hipStreamBeginCapture(s->compute, hipStreamCaptureModeThreadLocal));
+        const hipError_t le = mb::launch_batch_frame(p, s->compute);
+        const hipError_t ce = hipStreamEndCapture(s->compute, &graph);
+        if (le != hipSuccess) return fail_hip(le, "batched frame capture");
+        if (ce != hipSuccess) return fail_hip(ce, "hipStreamEndCapture");
+        const hipError_t ie = hipGraphInstantiate(&s->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) return fail_hip(ie, "hipGraphInstantiate");
+    }
+    s->dirty = false;
+    return MB_OK;
+}
+
+// Enqueues the copies of a finished scene's results on the copy stream.
+int retire(mb_server *s, Request *r)
+{
+    r->ev_frame = s->take_event();
+    if (!r->ev_frame) return fail(MB_ERR_HIP, "event creation failed");
+    MB_HIP(hipEventRecord(r->ev_frame, s->compute));
+    MB_HIP(hipStreamWaitEvent(s->copy, r->ev_frame, 0));
+    const mb::SceneDesc d = r->desc();
+    // the last frame ran at time_next - 1 and wrote state[!(t & 1)] = state[time_next & 1]
+    const int cur = r->time_next & 1;
+    const size_t n = r->n;
+    char *out = static_cast<char *>(r->io);  // its upload completed before the scene ran
+    MB_HIP(hipMemcpyAsync(out, d.pos[cur], n * 16, hipMemcpyDeviceToHost, s->copy));
+    MB_HIP(hipMemcpyAsync(out + n * 16, d.vfme[cur], n * 16, hipMemcpyDeviceToHost, s->copy));
+    if (r->keep_debug) {
+        MB_HIP(hipMemcpyAsync(out + n * 32, d.perm, n * 4, hipMemcpyDeviceToHost, s->copy));
+        MB_HIP(hipMemcpyAsync(out + n * 36, d.keys, n * 4, hipMemcpyDeviceToHost, s->copy));
+    }
+    r->ev_done = nullptr;
+    MB_HIP(hipEventCreateWithFlags(&r->ev_done, hipEventDisableTiming));
+    MB_HIP(hipEventRecord(r->ev_done, s->copy));
+    r->state = MB_REQUEST_RETIRING;
+    s->retiring.push_back(r);
+    return MB_OK;
+}
+
+int drain_timing(mb_server *s)
+{
+    for (auto &t : s->timed) {
+        MB_HIP(hipEventSynchronize(t.second));
+        float ms = 0.0f;
+        MB_HIP(hipEventElapsedTime(&ms, t.first, t.second));
+        s->frame_ms.push_back(ms);
+        (void)hipEventDestroy(t.first);
+        (void)hipEventDestroy(t.second);
+    }
+    s->timed.clear();
+    return MB_OK;
+}
+
+// Sets up what the first frames would otherwise set up while other work is
+// in flight: the plan buffers (room for 64 K scenes' worth of work lists
+// before they grow) and one large transfer each way on the copy stream (the
+// runtime's first large device-to-host copy blocks the calling thread).
+int warm_up(mb_server *s)
+{
+    constexpr size_t kPlanBytes = 8u << 20;
+    constexpr size_t kWarmBytes = 32u << 20;
+    MB_HIP(hipMalloc(&s->d_plan, kPlanBytes));
+    s->plan_cap = kPlanBytes;
+    for (int k = 0; k < 2; ++k) {
+        MB_HIP(hipHostMalloc(&s->stage[k], kPlanBytes, hipHostMallocMapped | hipHostMallocCoherent));
+        s->stage_bytes[k] = kPlanBytes;
+    }
+    size_t cls = 0;
+    void *host = s->pinned.get(kWarmBytes, cls);
+    if (!host) return fail(MB_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
+    void *dev = nullptr;
+    MB_HIP(hipMalloc(&dev, kWarmBytes));
+    std::memset(host, 0, kWarmBytes);
+    hipError_t e = hipMemcpyAsync(dev, host, kWarmBytes, hipMemcpyHostToDevice, s->copy);
+    if (e == hipSuccess) e = hipMemcpyAsync(host, dev, kWarmBytes, hipMemcpyDeviceToHost, s->copy);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->copy);
+    (void)hipFree(dev);
+    s->pinned.put(host, cls);
+    if (e != hipSuccess) return fail_hip(e, "transfer warm-up");
+    return MB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t mb_server_scene_bytes(uint32_t n, uint32_t g) { return scene_layout(n, g).bytes; }
+
+int mb_server_create(const mb_server_config *config, mb_server **out)
+{
+    if (!config || !out) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (config->memory_cap_bytes == 0)
+        return fail(MB_ERR_INVALID_ARGUMENT, "memory_cap_bytes must be > 0");
+    mb_server *s = new (std::nothrow) mb_server();
+    if (!s) return fail(MB_ERR_OUT_OF_MEMORY, "host allocation failed");
+    s->config = *config;
+    s->arena.reset(new Arena(config->chunk_bytes));
+    if (hipGetDevice(&s->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->compute, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess) {
+        delete s;
+        return fail(MB_ERR_HIP, "stream creation failed");
+    }
+    int rc = warm_up(s);
+    if (rc != MB_OK) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return MB_OK;
+}
+
+int mb_server_destroy(mb_server *s)
+{
+    delete s;
+    return MB_OK;
+}
+
+int mb_server_submit(mb_server *s, const mb_settings *settings, const float *pos,
+                     const uint32_t *vfme, uint32_t steps, int keep_debug, uint64_t *id)
+{
+    if (!s || !settings || !pos || !vfme || !id) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = validate_scene(*settings);
+    if (rc != MB_OK) return rc;
+    const uint32_t n = settings->max_number_of_agents;
+    const uint32_t g = settings->number_of_grids_one_dimension;
+    const SceneLayout layout = scene_layout(n, g);
+    if (layout.bytes > s->config.memory_cap_bytes)
+        return fail(MB_ERR_INVALID_ARGUMENT, "scene needs more device memory than the cap");
+    for (size_t i = 0; i < (size_t)n * 4u; ++i) {
+        if (!std::isfinite(pos[i]))
+            return fail(MB_ERR_INVALID_ARGUMENT, "non-finite position_with_scale at float " + std::to_string(i));
+        if (!finite_half(vfme[i] & 0xFFFFu) || !finite_half(vfme[i] >> 16))
+            return fail(MB_ERR_INVALID_ARGUMENT, "non-finite half in packed_vfme at word " + std::to_string(i));
+    }
+    std::unique_ptr<Request> r(new (std::nothrow) Request());
+    if (!r) return fail(MB_ERR_OUT_OF_MEMORY, "host allocation failed");
+    r->id = s->next_id++;
+    r->settings = *settings;
+    r->consts = mb::compile_consts(*settings);
+    r->n = n;
+    r->g = g;
+    r->steps = steps;
+    r->keep_debug = keep_debug != 0;
+    r->layout = layout;
+    r->io = s->pinned.get(r->out_bytes(), r->io_cls);
+    if (!r->io) return fail(MB_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
+    std::memcpy(r->io, pos, (size_t)n * 16);
+    std::memcpy(static_cast<char *>(r->io) + (size_t)n * 16, vfme, (size_t)n * 16);
+    r->submit_s = now_s();
+    *id = r->id;
+    s->queue.push_back(r.get());
+    s->requests.emplace(r->id, std::move(r));
+    return MB_OK;
+}
+
+int mb_server_tick(mb_server *s, uint32_t *running)
+{
+    if (!s) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    if (running) *running = 0;
+    int rc = poll_retiring(s);
+    if (rc != MB_OK) return rc;
+    rc = admit(s);
+    if (rc != MB_OK) return rc;
+    rc = join_uploaded(s);
+    if (rc != MB_OK) return rc;
+    if (s->running.empty()) return MB_OK;
+    // keep at most kDepth frames in flight so that the host does not run ahead
+    while (s->inflight.size() >= kDepth) {
+        MB_HIP(hipEventSynchronize(s->inflight.front()));
+        s->spare_events.push_back(s->inflight.front());
+        s->inflight.pop_front();
+    }
+    if (s->dirty) {
+        rc = upload_plan(s);
+        if (rc != MB_OK) return rc;
+    }
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (s->config.profile_frames) {
+        MB_HIP(hipEventCreate(&t0));
+        MB_HIP(hipEventCreate(&t1));
+        MB_HIP(hipEventRecord(t0, s->compute));
+    }
+    if (s->config.use_graph) {
+        MB_HIP(hipGraphLaunch(s->exec, s->compute));
+    } else {
+        hipError_t e = mb::launch_batch_frame(s->plan, s->compute);
+        if (e != hipSuccess) return fail_hip(e, "batched frame");
+    }
+    if (t1) {
+        MB_HIP(hipEventRecord(t1, s->compute));
+        s->timed.emplace_back(t0, t1);
+    }
+    hipEvent_t fe = s->take_event();
+    if (!fe) return fail(MB_ERR_HIP, "event creation failed");
+    MB_HIP(hipEventRecord(fe, s->compute));
+    s->inflight.push_back(fe);
+    ++s->frames;
+    if (running) *running = (uint32_t)s->running.size();
+
+    std::vector<Request *> still;
+    still.reserve(s->running.size());
+    for (Request *r : s->running) {
+        if (r->time_next > 0) s->sphere_steps += r->n;
+        r->time_next += 1;
+        r->frames_left -= 1;
+        r->frames_done += 1;
+        if (r->frames_left == 0) {
+            rc = retire(s, r);
+            if (rc != MB_OK) return rc;
+            s->dirty = true;
+        } else {
+            still.push_back(r);
+        }
+    }
+    s->running.swap(still);
+    return MB_OK;
+}
+
+int mb_server_drain(mb_server *s)
+{
+    if (!s) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    while (!s->queue.empty() || !s->uploading.empty() || !s->running.empty()) {
+        int rc = mb_server_tick(s, nullptr);
+        if (rc != MB_OK) return rc;
+        if (!s->running.empty()) continue;
+        if (!s->uploading.empty()) {
+            // nothing to step until the oldest upload lands
+            MB_HIP(hipEventSynchronize(s->uploading.front()->ev_upload));
+        } else if (!s->queue.empty()) {
+            // nothing runs and the head does not fit: wait for retirements
+            if (s->retiring.empty())
+                return fail(MB_ERR_OUT_OF_MEMORY, "queued scene cannot be admitted");
+            MB_HIP(hipStreamSynchronize(s->copy));
+        }
+    }
+    MB_HIP(hipStreamSynchronize(s->compute));
+    MB_HIP(hipStreamSynchronize(s->copy));
+    return poll_retiring(s);
+}
+
+int mb_server_status(mb_server *s, uint64_t id, mb_request_status *out)
+{
+    if (!s || !out) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    auto it = s->requests.find(id);
+    if (it == s->requests.end()) return fail(MB_ERR_INVALID_ARGUMENT, "unknown request id");
+    const Request *r = it->second.get();
+    out->state = r->state;
+    out->number_of_agents = r->n;
+    out->steps = r->steps;
+    out->frames_done = r->frames_done;
+    out->submit_s = r->submit_s;
+    out->admit_s = r->admit_s;
+    out->done_s = r->done_s;
+    return MB_OK;
+}
+
+int mb_server_collect(mb_server *s, uint64_t id, float *pos, uint32_t *vfme, uint32_t *perm,
+                      uint32_t *next_keys)
+{
+    if (!s) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    auto it = s->requests.find(id);
+    if (it == s->requests.end()) return fail(MB_ERR_INVALID_ARGUMENT, "unknown request id");
+    Request *r = it->second.get();
+    if (r->state == MB_REQUEST_QUEUED || r->state == MB_REQUEST_RUNNING)
+        return fail(MB_ERR_STATE, "request has not finished");
+    if (r->state == MB_REQUEST_RETIRING) {
+        MB_HIP(hipEventSynchronize(r->ev_done));
+        finish_retirement(s, r);
+        s->retiring.erase(std::find(s->retiring.begin(), s->retiring.end(), r));
+    }
+    if ((perm || next_keys) && !r->keep_debug)
+        return fail(MB_ERR_STATE, "request was submitted without keep_debug");
+    const size_t n = r->n;
+    const char *o = static_cast<const char *>(r->io);
+    if (pos) std::memcpy(pos, o, n * 16);
+    if (vfme) std::memcpy(vfme, o + n * 16, n * 16);
+    if (perm) std::memcpy(perm, o + n * 32, n * 4);
+    if (next_keys) std::memcpy(next_keys, o + n * 36, n * 4);
+    s->pinned.put(r->io, r->io_cls);
+    if (r->ev_done) (void)hipEventDestroy(r->ev_done);
+    s->requests.erase(it);
+    return MB_OK;
+}
+
+int mb_server_stats_read(const mb_server *s, mb_server_stats *out)
+{
+    if (!s || !out) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    out->queued = (uint32_t)s->queue.size();
+    out->running = (uint32_t)(s->running.size() + s->uploading.size());
+    out->retiring = (uint32_t)s->retiring.size();
+    uint32_t done = 0;
+    for (auto &kv : s->requests) done += kv.second->state == MB_REQUEST_DONE;
+    out->done = done;
+    out->used_bytes = s->used;
+    out->cap_bytes = s->config.memory_cap_bytes;
+    out->reserved_bytes = s->arena->reserved();
+    out->frames = s->frames;
+    out->sphere_steps = s->sphere_steps;
+    return MB_OK;
+}
+
+int mb_server_frame_times(mb_server *s, double *ms, uint32_t capacity, uint32_t *frames)
+{
+    if (!s || !frames) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = drain_timing(s);
+    if (rc != MB_OK) return rc;
+    *frames = (uint32_t)s->frame_ms.size();
+    const size_t count = ms ? std::min<size_t>(capacity, s->frame_ms.size()) : 0;
+    std::copy(s->frame_ms.begin(), s->frame_ms.begin() + (long)count, ms);
+    s->frame_ms.erase(s->frame_ms.begin(), s->frame_ms.begin() + (long)count);
+    return MB_OK;
+}
+
+}  // extern "C"

@@ -172,6 +172,93 @@ int mb_physics_run(const mb_settings *settings, int init, const uint32_t *perm,
                    void *position_with_scale_out, void *packed_vfme_out,
                    uint32_t number_of_agents, void *stream);
 
+/* ---- Serving: many independent scenes stepped together -----------------
+ *
+ * No counterpart in the reference, which steps one scene per app.  A server
+ * keeps many scenes (each with its own settings, sphere count and grid)
+ * resident in device memory and runs one frame of every resident scene per
+ * tick, in one set of launches.  Requests are queued, admitted first come,
+ * first served while their device memory fits under the cap, run INIT plus
+ * `steps` LOOP frames, and are retired into host memory as they finish, so
+ * requests that arrive while others run join the next frame.  A scene's
+ * results are the same alone and in any mix: every key, cell, row and slot
+ * is scene-local.  Scenes may use grids with G^2 > N (the single-scene rule
+ * G^2 <= N comes from the reference's buffer reuse, which a scene here does
+ * not share).  Not thread safe; all GPU work runs on the server's own
+ * streams. */
+
+typedef struct mb_server mb_server;
+
+typedef struct mb_server_config {
+    uint64_t memory_cap_bytes; /* admission cap over the scenes' device memory */
+    uint64_t chunk_bytes;      /* device arena growth step (0: 1 GiB) */
+    int32_t use_graph;         /* replay each batch composition as a captured hipGraph */
+    int32_t profile_frames;    /* record each frame's device time (HIP events) */
+} mb_server_config;
+
+#define MB_REQUEST_QUEUED 0
+#define MB_REQUEST_RUNNING 1   /* admitted: its upload or its frames are under way */
+#define MB_REQUEST_RETIRING 2 /* last frame enqueued, results on their way to host memory */
+#define MB_REQUEST_DONE 3
+
+typedef struct mb_request_status {
+    int32_t state;             /* MB_REQUEST_* */
+    uint32_t number_of_agents;
+    uint32_t steps;            /* LOOP frames requested */
+    uint32_t frames_done;      /* frames enqueued so far, INIT included */
+    double submit_s, admit_s, done_s; /* steady-clock seconds; 0 until reached */
+} mb_request_status;
+
+typedef struct mb_server_stats {
+    uint32_t queued, running, retiring, done;
+    uint64_t used_bytes, cap_bytes, reserved_bytes;
+    uint64_t frames;           /* ticks that ran a frame */
+    double sphere_steps;       /* sum over LOOP frames of the scene sizes */
+} mb_server_stats;
+
+/* Bytes of device memory one scene of n spheres on a G x G grid holds. */
+uint64_t mb_server_scene_bytes(uint32_t number_of_agents, uint32_t number_of_grids_one_dimension);
+
+int mb_server_create(const mb_server_config *config, mb_server **out);
+/* Waits for the server's streams, then frees everything. */
+int mb_server_destroy(mb_server *server);
+
+/* Queues a scene: settings (max_number_of_agents = its sphere count), its
+ * initial state on the host (copied before returning) and the number of LOOP
+ * frames to run after INIT.  keep_debug != 0 also returns the perm and next
+ * keys of its last frame.  Rejects what mb_physics_create rejects except
+ * G^2 > N, and scenes larger than the memory cap. */
+int mb_server_submit(mb_server *server, const mb_settings *settings,
+                     const float *position_with_scale, const uint32_t *packed_vfme,
+                     uint32_t steps, int keep_debug, uint64_t *request_id);
+
+/* One scheduling round: completes retirements whose copies finished,
+ * admits queued requests that fit, runs one frame of every resident scene
+ * and retires the scenes whose last frame that was.  *running receives the
+ * number of scenes in the frame (0: nothing ran).  Does not wait for the
+ * GPU, except that at most `depth` (2) frames are kept in flight. */
+int mb_server_tick(mb_server *server, uint32_t *running);
+
+/* Ticks until no request is queued or running and every retirement has
+ * reached host memory. */
+int mb_server_drain(mb_server *server);
+
+int mb_server_status(mb_server *server, uint64_t request_id, mb_request_status *out);
+
+/* Copies a finished request's final state (and, when kept, the perm and next
+ * keys of its last frame; any pointer may be NULL) and forgets the request.
+ * Waits for its retirement copy if needed; MB_ERR_STATE if it is still
+ * queued or running. */
+int mb_server_collect(mb_server *server, uint64_t request_id, float *position_with_scale,
+                      uint32_t *packed_vfme, uint32_t *perm, uint32_t *next_keys);
+
+int mb_server_stats_read(const mb_server *server, mb_server_stats *out);
+
+/* Device time of each frame not read yet (profile_frames on), in ms: waits
+ * for those frames, stores their count in *frames, moves the first
+ * `capacity` of them to `ms` (NULL: none) and keeps the rest. */
+int mb_server_frame_times(mb_server *server, double *ms, uint32_t capacity, uint32_t *frames);
+
 #ifdef __cplusplus
 }
 #endif

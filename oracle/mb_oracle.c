@@ -607,7 +607,31 @@ struct mbo_world {
     uint32_t *vfme[2];
     uint32_t *keys[2]; /* EVE/ODD global keys (sort input, start, next keys) */
     uint32_t *vals[2]; /* EVE/ODD global vals (perm, close) */
+    size_t cells;      /* entries of keys/vals: max(N, G^2) */
 };
+
+/* The reference sizes the start/close arrays by N (they reuse the sort
+ * buffers), hence its rule G^2 <= N (mod.rs:351-362), and clears N entries of
+ * start.  A scene of the serving path may have G^2 > N; its arrays here have
+ * max(N, G^2) entries and all of them are cleared, which is the reference's
+ * behaviour whenever G^2 <= N. */
+static size_t world_cells(size_t n, const mbo_settings *s)
+{
+    size_t g = s->number_of_grids_one_dimension;
+    return g * g > n ? g * g : n;
+}
+
+static void world_grow(mbo_world *w, size_t cells)
+{
+    if (cells <= w->cells) return;
+    for (int p = 0; p < 2; ++p) {
+        w->keys[p] = (uint32_t *)realloc(w->keys[p], (cells + 1u) * sizeof(uint32_t));
+        w->vals[p] = (uint32_t *)realloc(w->vals[p], (cells + 1u) * sizeof(uint32_t));
+        memset(w->keys[p] + w->cells + 1u, 0, (cells - w->cells) * sizeof(uint32_t));
+        memset(w->vals[p] + w->cells + 1u, 0, (cells - w->cells) * sizeof(uint32_t));
+    }
+    w->cells = cells;
+}
 
 mbo_world *mbo_world_create(const mbo_settings *s)
 {
@@ -623,6 +647,8 @@ mbo_world *mbo_world_create(const mbo_settings *s)
         w->keys[p] = (uint32_t *)calloc(n + 1u, sizeof(uint32_t));
         w->vals[p] = (uint32_t *)calloc(n + 1u, sizeof(uint32_t));
     }
+    w->cells = n;
+    world_grow(w, world_cells(n, s));
     return w;
 }
 
@@ -639,6 +665,7 @@ void mbo_world_set_consts(mbo_world *w, const mbo_settings *s)
 {
     w->s = *s;
     mbo_compile_consts(s, &w->c);
+    world_grow(w, world_cells(w->n, s));
 }
 
 /* Buffers as created by create_shader_storage_buffers (mod.rs:241-292): the
@@ -678,7 +705,8 @@ void mbo_world_step(mbo_world *w, uint32_t steps, int threads, double *stage_sec
         mbo_radix_sort(w->keys[p], NULL, w->keys[q], w->vals[q], w->n, 0u, 3u, threads);
         double t1 = now_seconds();
         /* spatial_hashing::run(.., N, !is_even): start -> keys[p], close -> vals[p] */
-        mbo_spatial_hashing(w->keys[q], w->keys[p], w->vals[p], w->n, w->n, threads);
+        mbo_spatial_hashing(w->keys[q], w->keys[p], w->vals[p], w->n, (uint32_t)w->cells,
+                            threads);
         double t2 = now_seconds();
         /* physics::run: INIT at time 0, else LOOP (mod.rs:929-939) */
         mbo_physics_run(&w->c, w->time == 0, w->vals[q], w->keys[p], w->vals[p],
@@ -708,8 +736,8 @@ void mbo_world_debug(const mbo_world *w, uint32_t *perm, uint32_t *start,
     int p = (int)(w->time & 1);
     int q = 1 - p;
     if (perm) memcpy(perm, w->vals[q], (size_t)w->n * 4u);
-    if (start) memcpy(start, w->keys[p], (size_t)w->n * 4u);
-    if (close) memcpy(close, w->vals[p], (size_t)w->n * 4u);
+    if (start) memcpy(start, w->keys[p], w->cells * 4u);
+    if (close) memcpy(close, w->vals[p], w->cells * 4u);
 }
 
 void mbo_world_next_keys(const mbo_world *w, uint32_t *keys)

@@ -113,7 +113,8 @@ void mbo_world_step(mbo_world *w, uint32_t steps, int threads,
 int64_t mbo_world_time(const mbo_world *w);
 void mbo_world_download(const mbo_world *w, float *pos_scale, uint32_t *vfme);
 /* Permutation and cell ranges the last frame used (start/close hold
- * max_number_of_agents entries, as the reference's buffers do). */
+ * max(max_number_of_agents, G^2) entries: the reference's N whenever
+ * G^2 <= N). */
 void mbo_world_debug(const mbo_world *w, uint32_t *perm, uint32_t *start,
                      uint32_t *close);
 /* Cell keys the last frame wrote for the next sort. */

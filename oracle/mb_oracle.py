@@ -246,7 +246,10 @@ class World:
         return pos, vf
 
     def debug(self):
-        out = {k: np.empty(self.n, np.uint32) for k in ("perm", "start", "close", "next_keys")}
+        g = int(self._s.number_of_grids_one_dimension)
+        cells = max(self.n, g * g)
+        out = {k: np.empty(self.n, np.uint32) for k in ("perm", "next_keys")}
+        out.update({k: np.empty(cells, np.uint32) for k in ("start", "close")})
         lib().mbo_world_debug(self._w, _p(out["perm"]), _p(out["start"]), _p(out["close"]))
         lib().mbo_world_next_keys(self._w, _p(out["next_keys"]))
         return out
