@@ -11,7 +11,7 @@
 //     advanced by the frame's last kernel, so an unchanged composition can
 //     be replayed as one captured hipGraph.  The host keeps a mirror of both
 //     counters and rewrites the table only when the composition changes.
-//   - Uploads and retirement downloads run on a second stream, so they
+//   - Uploads and retirement downloads run on two more streams, so they
 //     overlap other scenes' frames.  An admitted scene joins the frames once
 //     its upload has completed; a retired scene's download waits for its
 //     last frame (an event), and its region returns to the arena only after
@@ -123,7 +123,19 @@ private:
 // Device arena: chunks from hipMalloc, first-fit regions, coalescing frees.
 class Arena {
 public:
-    explicit Arena(uint64_t chunk) : chunk_(chunk ? chunk : (1ull << 30)) {}
+    explicit Arena(uint64_t chunk) : chunk_(chunk) {}
+    // Reserves the first chunk up front (hipMalloc of a large range takes
+    // milliseconds, which a running server should not spend mid-stream).
+    bool reserve()
+    {
+        Chunk c;
+        c.bytes = chunk_;
+        if (hipMalloc(&c.base, c.bytes) != hipSuccess) return false;
+        c.free[0] = c.bytes;
+        chunks_.push_back(c);
+        reserved_ += c.bytes;
+        return true;
+    }
     ~Arena()
     {
         for (auto &c : chunks_) (void)hipFree(c.base);
@@ -249,7 +261,9 @@ int validate_scene(const mb_settings &s)
 struct mb_server {
     mb_server_config config{};
     int device = 0;
-    hipStream_t compute = nullptr, copy = nullptr;
+    // frames; host->device uploads (never wait on other streams); device->host
+    // retirements (each waits for its scene's last frame)
+    hipStream_t compute = nullptr, copy = nullptr, down = nullptr;
     std::unique_ptr<Arena> arena;
     PinnedPool pinned;
     uint64_t used = 0;
@@ -257,7 +271,8 @@ struct mb_server {
     std::deque<Request *> queue;
     std::deque<Request *> uploading;  // admitted, state still on its way to the device
     std::vector<Request *> running;   // plan order
-    std::vector<Request *> retiring;
+    std::vector<Request *> finishing; // last frame enqueued, not yet complete
+    std::vector<Request *> retiring;  // results on their way to host memory
     std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
     bool dirty = true;
 
@@ -282,6 +297,7 @@ struct mb_server {
     std::deque<hipEvent_t> inflight;
     std::vector<hipEvent_t> spare_events;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> timed;
+    std::vector<hipEvent_t> spare_timing;  // timing events are costly to create: reused
     std::vector<double> frame_ms;
 
     uint64_t frames = 0;
@@ -291,6 +307,7 @@ struct mb_server {
     {
         if (compute) (void)hipStreamSynchronize(compute);
         if (copy) (void)hipStreamSynchronize(copy);
+        if (down) (void)hipStreamSynchronize(down);
         for (auto &kv : requests) {
             Request *r = kv.second.get();
             if (r->ev_upload) (void)hipEventDestroy(r->ev_upload);
@@ -309,6 +326,7 @@ struct mb_server {
             (void)hipEventDestroy(t.first);
             (void)hipEventDestroy(t.second);
         }
+        for (hipEvent_t e : spare_timing) (void)hipEventDestroy(e);
         for (int k = 0; k < 2; ++k) {
             if (stage[k]) (void)hipHostFree(stage[k]);
             if (stage_ev[k]) (void)hipEventDestroy(stage_ev[k]);
@@ -319,6 +337,7 @@ struct mb_server {
         arena.reset();
         if (compute) (void)hipStreamDestroy(compute);
         if (copy) (void)hipStreamDestroy(copy);
+        if (down) (void)hipStreamDestroy(down);
     }
 
     hipEvent_t take_event()
@@ -344,8 +363,8 @@ void finish_retirement(mb_server *s, Request *r)
     r->region = nullptr;
     r->state = MB_REQUEST_DONE;
     r->done_s = now_s();
-    // both events have been waited on by work that is now complete
-    for (hipEvent_t *e : {&r->ev_upload, &r->ev_frame}) {
+    // every event of the request has completed
+    for (hipEvent_t *e : {&r->ev_upload, &r->ev_frame, &r->ev_done}) {
         if (*e) s->spare_events.push_back(*e);
         *e = nullptr;
     }
@@ -379,7 +398,13 @@ int poll_retiring(mb_server *s)
 
 int admit(mb_server *s)
 {
-    while (!s->queue.empty()) {
+    // Admission enqueues the scene's upload; keeping about kUploadBytes in
+    // flight feeds the copy engine without holding up the frame launches of
+    // this tick behind a long run of copy calls.
+    constexpr uint64_t kUploadBytes = 64ull << 20;
+    uint64_t in_flight = 0;
+    for (const Request *r : s->uploading) in_flight += (uint64_t)r->n * 32;
+    while (!s->queue.empty() && (in_flight < kUploadBytes || s->uploading.empty())) {
         Request *r = s->queue.front();
         if (s->used + r->layout.bytes > s->config.memory_cap_bytes) break;
         char *region = s->arena->alloc(r->layout.bytes);
@@ -400,6 +425,7 @@ int admit(mb_server *s)
         r->time_next = 0;
         r->frames_left = r->steps + 1;
         s->uploading.push_back(r);
+        in_flight += (uint64_t)r->n * 32;
     }
     return MB_OK;
 }
@@ -512,45 +538,74 @@ int upload_plan(mb_server *s)
     return MB_OK;
 }
 
-// Enqueues the copies of a finished scene's results on the copy stream.
+// A scene's last frame has been enqueued: remember when it completes.  Its
+// download is enqueued only once that frame is done (start_downloads), so the
+// copy never waits on another stream inside the runtime.
 int retire(mb_server *s, Request *r)
 {
     r->ev_frame = s->take_event();
     if (!r->ev_frame) return fail(MB_ERR_HIP, "event creation failed");
     MB_HIP(hipEventRecord(r->ev_frame, s->compute));
-    MB_HIP(hipStreamWaitEvent(s->copy, r->ev_frame, 0));
-    const mb::SceneDesc d = r->desc();
-    // the last frame ran at time_next - 1 and wrote state[!(t & 1)] = state[time_next & 1]
-    const int cur = r->time_next & 1;
-    const size_t n = r->n;
-    char *out = static_cast<char *>(r->io);  // its upload completed before the scene ran
-    MB_HIP(hipMemcpyAsync(out, d.pos[cur], n * 16, hipMemcpyDeviceToHost, s->copy));
-    MB_HIP(hipMemcpyAsync(out + n * 16, d.vfme[cur], n * 16, hipMemcpyDeviceToHost, s->copy));
-    if (r->keep_debug) {
-        MB_HIP(hipMemcpyAsync(out + n * 32, d.perm, n * 4, hipMemcpyDeviceToHost, s->copy));
-        MB_HIP(hipMemcpyAsync(out + n * 36, d.keys, n * 4, hipMemcpyDeviceToHost, s->copy));
-    }
-    r->ev_done = nullptr;
-    MB_HIP(hipEventCreateWithFlags(&r->ev_done, hipEventDisableTiming));
-    MB_HIP(hipEventRecord(r->ev_done, s->copy));
     r->state = MB_REQUEST_RETIRING;
-    s->retiring.push_back(r);
+    s->finishing.push_back(r);
     return MB_OK;
 }
 
-int drain_timing(mb_server *s)
+int start_downloads(mb_server *s)
 {
-    for (auto &t : s->timed) {
-        MB_HIP(hipEventSynchronize(t.second));
+    for (size_t i = 0; i < s->finishing.size();) {
+        Request *r = s->finishing[i];
+        const hipError_t q = hipEventQuery(r->ev_frame);
+        if (q == hipErrorNotReady) {
+            ++i;
+            continue;
+        }
+        if (q != hipSuccess) return fail_hip(q, "last frame");
+        const mb::SceneDesc d = r->desc();
+        // the last frame ran at time_next - 1 and wrote state[!(t & 1)] = state[time_next & 1]
+        const int cur = r->time_next & 1;
+        const size_t n = r->n;
+        char *out = static_cast<char *>(r->io);  // its upload completed before the scene ran
+        MB_HIP(hipMemcpyAsync(out, d.pos[cur], n * 16, hipMemcpyDeviceToHost, s->down));
+        MB_HIP(hipMemcpyAsync(out + n * 16, d.vfme[cur], n * 16, hipMemcpyDeviceToHost, s->down));
+        if (r->keep_debug) {
+            MB_HIP(hipMemcpyAsync(out + n * 32, d.perm, n * 4, hipMemcpyDeviceToHost, s->down));
+            MB_HIP(hipMemcpyAsync(out + n * 36, d.keys, n * 4, hipMemcpyDeviceToHost, s->down));
+        }
+        r->ev_done = s->take_event();
+        if (!r->ev_done) return fail(MB_ERR_HIP, "event creation failed");
+        MB_HIP(hipEventRecord(r->ev_done, s->down));
+        s->retiring.push_back(r);
+        s->finishing.erase(s->finishing.begin() + (long)i);
+    }
+    return MB_OK;
+}
+
+// Moves the durations of completed frames into frame_ms (all of them when
+// `wait`, else only those whose end event has completed).
+int collect_timing(mb_server *s, bool wait)
+{
+    size_t done = 0;
+    for (; done < s->timed.size(); ++done) {
+        auto &t = s->timed[done];
+        if (wait) {
+            MB_HIP(hipEventSynchronize(t.second));
+        } else {
+            const hipError_t q = hipEventQuery(t.second);
+            if (q == hipErrorNotReady) break;
+            if (q != hipSuccess) return fail_hip(q, "frame timing");
+        }
         float ms = 0.0f;
         MB_HIP(hipEventElapsedTime(&ms, t.first, t.second));
         s->frame_ms.push_back(ms);
-        (void)hipEventDestroy(t.first);
-        (void)hipEventDestroy(t.second);
+        s->spare_timing.push_back(t.first);
+        s->spare_timing.push_back(t.second);
     }
-    s->timed.clear();
+    s->timed.erase(s->timed.begin(), s->timed.begin() + (long)done);
     return MB_OK;
 }
+
+int drain_timing(mb_server *s) { return collect_timing(s, true); }
 
 // Sets up what the first frames would otherwise set up while other work is
 // in flight: the plan buffers (room for 64 K scenes' worth of work lists
@@ -573,8 +628,9 @@ int warm_up(mb_server *s)
     MB_HIP(hipMalloc(&dev, kWarmBytes));
     std::memset(host, 0, kWarmBytes);
     hipError_t e = hipMemcpyAsync(dev, host, kWarmBytes, hipMemcpyHostToDevice, s->copy);
-    if (e == hipSuccess) e = hipMemcpyAsync(host, dev, kWarmBytes, hipMemcpyDeviceToHost, s->copy);
     if (e == hipSuccess) e = hipStreamSynchronize(s->copy);
+    if (e == hipSuccess) e = hipMemcpyAsync(host, dev, kWarmBytes, hipMemcpyDeviceToHost, s->down);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->down);
     (void)hipFree(dev);
     s->pinned.put(host, cls);
     if (e != hipSuccess) return fail_hip(e, "transfer warm-up");
@@ -596,14 +652,17 @@ int mb_server_create(const mb_server_config *config, mb_server **out)
     mb_server *s = new (std::nothrow) mb_server();
     if (!s) return fail(MB_ERR_OUT_OF_MEMORY, "host allocation failed");
     s->config = *config;
-    s->arena.reset(new Arena(config->chunk_bytes));
+    const uint64_t chunk = config->chunk_bytes ? config->chunk_bytes : (4ull << 30);
+    s->arena.reset(new Arena(std::min<uint64_t>(align_up(chunk), align_up(config->memory_cap_bytes))));
     if (hipGetDevice(&s->device) != hipSuccess ||
         hipStreamCreateWithFlags(&s->compute, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess) {
+        hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->down, hipStreamNonBlocking) != hipSuccess) {
         delete s;
         return fail(MB_ERR_HIP, "stream creation failed");
     }
-    int rc = warm_up(s);
+    int rc = s->arena->reserve() ? warm_up(s)
+                                 : fail(MB_ERR_OUT_OF_MEMORY, "device arena reservation failed");
     if (rc != MB_OK) {
         delete s;
         return rc;
@@ -662,6 +721,8 @@ int mb_server_tick(mb_server *s, uint32_t *running)
     if (running) *running = 0;
     int rc = poll_retiring(s);
     if (rc != MB_OK) return rc;
+    rc = start_downloads(s);
+    if (rc != MB_OK) return rc;
     rc = admit(s);
     if (rc != MB_OK) return rc;
     rc = join_uploaded(s);
@@ -673,14 +734,24 @@ int mb_server_tick(mb_server *s, uint32_t *running)
         s->spare_events.push_back(s->inflight.front());
         s->inflight.pop_front();
     }
+    if (s->config.profile_frames) {
+        rc = collect_timing(s, false);
+        if (rc != MB_OK) return rc;
+    }
     if (s->dirty) {
         rc = upload_plan(s);
         if (rc != MB_OK) return rc;
     }
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if (s->config.profile_frames) {
-        MB_HIP(hipEventCreate(&t0));
-        MB_HIP(hipEventCreate(&t1));
+        for (hipEvent_t *e : {&t0, &t1}) {
+            if (!s->spare_timing.empty()) {
+                *e = s->spare_timing.back();
+                s->spare_timing.pop_back();
+            } else {
+                MB_HIP(hipEventCreate(e));
+            }
+        }
         MB_HIP(hipEventRecord(t0, s->compute));
     }
     if (s->config.use_graph) {
@@ -731,13 +802,19 @@ int mb_server_drain(mb_server *s)
             MB_HIP(hipEventSynchronize(s->uploading.front()->ev_upload));
         } else if (!s->queue.empty()) {
             // nothing runs and the head does not fit: wait for retirements
-            if (s->retiring.empty())
+            if (s->finishing.empty() && s->retiring.empty())
                 return fail(MB_ERR_OUT_OF_MEMORY, "queued scene cannot be admitted");
-            MB_HIP(hipStreamSynchronize(s->copy));
+            MB_HIP(hipStreamSynchronize(s->compute));
+            int rc2 = start_downloads(s);
+            if (rc2 != MB_OK) return rc2;
+            MB_HIP(hipStreamSynchronize(s->down));
         }
     }
     MB_HIP(hipStreamSynchronize(s->compute));
+    int rc = start_downloads(s);
+    if (rc != MB_OK) return rc;
     MB_HIP(hipStreamSynchronize(s->copy));
+    MB_HIP(hipStreamSynchronize(s->down));
     return poll_retiring(s);
 }
 
@@ -767,6 +844,11 @@ int mb_server_collect(mb_server *s, uint64_t id, float *pos, uint32_t *vfme, uin
     if (r->state == MB_REQUEST_QUEUED || r->state == MB_REQUEST_RUNNING)
         return fail(MB_ERR_STATE, "request has not finished");
     if (r->state == MB_REQUEST_RETIRING) {
+        if (std::find(s->finishing.begin(), s->finishing.end(), r) != s->finishing.end()) {
+            MB_HIP(hipEventSynchronize(r->ev_frame));
+            int rc = start_downloads(s);
+            if (rc != MB_OK) return rc;
+        }
         MB_HIP(hipEventSynchronize(r->ev_done));
         finish_retirement(s, r);
         s->retiring.erase(std::find(s->retiring.begin(), s->retiring.end(), r));
@@ -780,7 +862,6 @@ int mb_server_collect(mb_server *s, uint64_t id, float *pos, uint32_t *vfme, uin
     if (perm) std::memcpy(perm, o + n * 32, n * 4);
     if (next_keys) std::memcpy(next_keys, o + n * 36, n * 4);
     s->pinned.put(r->io, r->io_cls);
-    if (r->ev_done) (void)hipEventDestroy(r->ev_done);
     s->requests.erase(it);
     return MB_OK;
 }
@@ -790,7 +871,7 @@ int mb_server_stats_read(const mb_server *s, mb_server_stats *out)
     if (!s || !out) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
     out->queued = (uint32_t)s->queue.size();
     out->running = (uint32_t)(s->running.size() + s->uploading.size());
-    out->retiring = (uint32_t)s->retiring.size();
+    out->retiring = (uint32_t)(s->finishing.size() + s->retiring.size());
     uint32_t done = 0;
     for (auto &kv : s->requests) done += kv.second->state == MB_REQUEST_DONE;
     out->done = done;

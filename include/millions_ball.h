@@ -191,7 +191,8 @@ typedef struct mb_server mb_server;
 
 typedef struct mb_server_config {
     uint64_t memory_cap_bytes; /* admission cap over the scenes' device memory */
-    uint64_t chunk_bytes;      /* device arena growth step (0: 1 GiB) */
+    uint64_t chunk_bytes;      /* device arena chunk, the first one reserved at creation
+                                  (0: 4 GiB; at most the cap) */
     int32_t use_graph;         /* replay each batch composition as a captured hipGraph */
     int32_t profile_frames;    /* record each frame's device time (HIP events) */
 } mb_server_config;
