@@ -34,19 +34,14 @@ HBM_PEAK_GBS = 8000.0
 def physics_bytes_per_sphere(n: int, cells: int) -> float:
     """Algorithmic HBM bytes of one collide + integrate launch per sphere:
     sorted key in 4 + perm 4 + state in 32 + cell offsets 4 * C / N
-    + state out 32 + next key out 4 (written over the sorted key)."""
-    return 76.0 + 4.0 * cells / n
-
-
-def partition_bytes_per_sphere() -> float:
-    """Row count (key in 4) + row partition (key in 4 + (column, index) pair
-    out 8): the first timed stage."""
-    return 16.0
+    + state out 32 + next key out 4 (written over the sorted key)
+    + (column, slot) pair filed in the next row's bucket 8."""
+    return 84.0 + 4.0 * cells / n
 
 
 def row_sort_bytes_per_sphere(n: int, cells: int) -> float:
-    """Row sort: pair in 8 + perm out 4 + sorted key out 4 + cell offsets
-    out 4 * C / N."""
+    """Row sort: bucket pair in 8 + perm out 4 + sorted key out 4 + cell
+    offsets out 4 * C / N."""
     return 16.0 + 4.0 * cells / n
 
 
@@ -251,9 +246,8 @@ def main() -> int:
     phys.close()
 
     phys_bytes = physics_bytes_per_sphere(n, cells) * n
-    achieved = phys_bytes / (stage_ms[2] * 1e-3) / 1e9
-    part_gbs = partition_bytes_per_sphere() * n / (stage_ms[0] * 1e-3) / 1e9
-    rsort_gbs = row_sort_bytes_per_sphere(n, cells) * n / (stage_ms[1] * 1e-3) / 1e9
+    achieved = phys_bytes / (stage_ms[1] * 1e-3) / 1e9
+    rsort_gbs = row_sort_bytes_per_sphere(n, cells) * n / (stage_ms[0] * 1e-3) / 1e9
 
     traffic = read_traffic(cfg.name)
 
@@ -291,10 +285,8 @@ def main() -> int:
         "download_ms": download_ms,
         "device_step_ms_p50": float(np.percentile(device_step_ms, 50)),
         "device_step_ms_p99": float(np.percentile(device_step_ms, 99)),
-        "stage_ms": {"row_partition": float(stage_ms[0]), "row_sort": float(stage_ms[1]),
-                     "collide_integrate": float(stage_ms[2])},
-        "stage_achieved_GBs": {"row_partition": part_gbs, "row_sort": rsort_gbs,
-                               "collide_integrate": achieved},
+        "stage_ms": {"row_sort": float(stage_ms[0]), "collide_integrate": float(stage_ms[1])},
+        "stage_achieved_GBs": {"row_sort": rsort_gbs, "collide_integrate": achieved},
         "roofline": {
             "kernel": "physics_step_kernel (collide + integrate)",
             "bound": "hbm",

@@ -148,21 +148,35 @@ struct mb_physics {
     uint32_t *keys[2] = {nullptr, nullptr};
     uint32_t *vals[2] = {nullptr, nullptr};
     uint32_t *off = nullptr;        // per-cell slot offsets, N + 1
-    uint32_t *rows = nullptr;       // row_count | row_fill | row_start
-    uint64_t *pairs = nullptr;      // N
+    // row buckets: counters (2 sets of kMaxRows + the two spill counts), the
+    // G x cap buckets (sized for the current grid) and the spill list
+    uint32_t *counters = nullptr;
+    uint64_t *bucket = nullptr;
+    uint32_t bucket_rows = 0, bucket_cap = 0;
+    uint64_t *spill = nullptr;          // N
+    uint64_t *scratch_pairs = nullptr;  // N
     uint32_t *scratch_idx = nullptr;    // N
-    // Profiling: four events per frame (before the row partition, before the
-    // row sort, before collide + integrate, after it) recorded without
-    // blocking; read back by mb_physics_profile_read.
+    // Profiling: three events per frame (before the row sort, before
+    // collide + integrate, after it) recorded without blocking; read back by
+    // mb_physics_profile_read.
     bool profiling = false;
-    std::vector<std::array<hipEvent_t, 4>> prof_events;
+    std::vector<std::array<hipEvent_t, 3>> prof_events;
     size_t prof_used = 0;
-    std::vector<double> prof_ms;  // [frame][3] of frames already read back
+    std::vector<double> prof_ms;  // [frame][2] of frames already read back
 
-    uint32_t *row_count() const { return rows; }
-    uint32_t *row_fill() const { return rows + mb::kMaxRows; }
-    uint32_t *row_start() const { return rows + 2 * mb::kMaxRows; }
+    mb::RowBuckets buckets() const
+    {
+        mb::RowBuckets rb;
+        rb.bucket = bucket;
+        rb.spill = spill;
+        rb.fill = counters;  // set-major, G counters per set
+        rb.spill_count = counters + 2 * (size_t)mb::kMaxRows * mb::kLine;
+        rb.cap = bucket_cap;
+        return rb;
+    }
 };
+
+static const size_t kCounterWords = mb::bucket_counter_words(mb::kMaxRows);
 
 extern "C" {
 
@@ -197,8 +211,10 @@ static void release(mb_physics *p)
         if (p->vals[i]) (void)hipFree(p->vals[i]);
     }
     if (p->off) (void)hipFree(p->off);
-    if (p->rows) (void)hipFree(p->rows);
-    if (p->pairs) (void)hipFree(p->pairs);
+    if (p->counters) (void)hipFree(p->counters);
+    if (p->bucket) (void)hipFree(p->bucket);
+    if (p->spill) (void)hipFree(p->spill);
+    if (p->scratch_pairs) (void)hipFree(p->scratch_pairs);
     if (p->scratch_idx) (void)hipFree(p->scratch_idx);
     for (auto &fe : p->prof_events)
         for (hipEvent_t e : fe)
@@ -206,13 +222,31 @@ static void release(mb_physics *p)
     p->prof_events.clear();
 }
 
+// (Re)allocates the row buckets for the current grid.
+static int alloc_buckets(mb_physics *p)
+{
+    const uint32_t g = p->settings.number_of_grids_one_dimension;
+    const uint32_t cap = mb::bucket_cap(p->n, g);
+    if (p->bucket && p->bucket_rows == g && p->bucket_cap == cap) return MB_OK;
+    if (p->bucket) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(p->bucket);
+        p->bucket = nullptr;
+    }
+    if (hipMalloc(&p->bucket, (size_t)g * cap * 8u) != hipSuccess)
+        return fail(MB_ERR_OUT_OF_MEMORY, "device allocation failed (row buckets)");
+    p->bucket_rows = g;
+    p->bucket_cap = cap;
+    return MB_OK;
+}
+
 // Moves the durations of every recorded frame into prof_ms.
 static int drain_profile(mb_physics *p)
 {
     if (p->prof_used == 0) return MB_OK;
-    MB_HIP(hipEventSynchronize(p->prof_events[p->prof_used - 1][3]));
+    MB_HIP(hipEventSynchronize(p->prof_events[p->prof_used - 1][2]));
     for (size_t i = 0; i < p->prof_used; ++i) {
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < 2; ++k) {
             float ms = 0.0f;
             MB_HIP(hipEventElapsedTime(&ms, p->prof_events[i][k], p->prof_events[i][k + 1]));
             p->prof_ms.push_back(ms);
@@ -238,7 +272,6 @@ int mb_physics_create(const mb_settings *settings, mb_physics **out)
         return fail(MB_ERR_HIP, "no HIP device");
     }
     const size_t n = p->n;
-    const size_t row_words = 3 * (size_t)mb::kMaxRows + 1;
     bool ok = true;
     for (int i = 0; i < 2 && ok; ++i) {
         ok = hipMalloc(&p->pos[i], n * sizeof(float4)) == hipSuccess &&
@@ -247,8 +280,10 @@ int mb_physics_create(const mb_settings *settings, mb_physics **out)
              hipMalloc(&p->vals[i], n * 4u) == hipSuccess;
     }
     ok = ok && hipMalloc(&p->off, (n + 1) * 4u) == hipSuccess;
-    ok = ok && hipMalloc(&p->rows, row_words * 4u) == hipSuccess;
-    ok = ok && hipMalloc(&p->pairs, n * 8u) == hipSuccess;
+    ok = ok && hipMalloc(&p->counters, kCounterWords * 4u) == hipSuccess;
+    ok = ok && alloc_buckets(p) == MB_OK;
+    ok = ok && hipMalloc(&p->spill, n * 8u) == hipSuccess;
+    ok = ok && hipMalloc(&p->scratch_pairs, n * 8u) == hipSuccess;
     ok = ok && hipMalloc(&p->scratch_idx, n * 4u) == hipSuccess;
     if (!ok) {
         release(p);
@@ -256,7 +291,7 @@ int mb_physics_create(const mb_settings *settings, mb_physics **out)
         return fail(MB_ERR_OUT_OF_MEMORY, "device allocation failed");
     }
     // Buffers start zeroed, as wgpu's do (the INIT frame sorts all-zero keys).
-    ok = hipMemset(p->rows, 0, row_words * 4u) == hipSuccess &&
+    ok = hipMemset(p->counters, 0, kCounterWords * 4u) == hipSuccess &&
          hipMemset(p->off, 0, (n + 1) * 4u) == hipSuccess;
     for (int i = 0; i < 2 && ok; ++i) {
         ok = hipMemset(p->pos[i], 0, n * sizeof(float4)) == hipSuccess &&
@@ -308,7 +343,11 @@ int mb_physics_set_settings(mb_physics *p, const mb_settings *s)
         // the stale keys of the old grid once, reading stale `close` words).
         DeviceGuard g(p->device);
         const int cur = (int)((p->time + 1) & 1);
-        hipError_t e = mb::launch_rekey(p->consts, p->pos[cur], p->keys[cur], p->n, nullptr);
+        int rc2 = alloc_buckets(p);
+        if (rc2 != MB_OK) return rc2;
+        // the next frame (time + 1) sorts bucket set (time + 1) & 1
+        hipError_t e = mb::launch_rekey(p->consts, p->pos[cur], p->keys[cur], p->buckets(),
+                                        (uint32_t)((p->time + 1) & 1), p->n, nullptr);
         if (e != hipSuccess) return fail_hip(e, "rekey");
         MB_HIP(hipStreamSynchronize(nullptr));
     }
@@ -345,7 +384,7 @@ int mb_physics_upload_state_device(mb_physics *p, const void *pos, const void *v
         MB_HIP(hipMemsetAsync(p->keys[i], 0, n * 4u, st));
         MB_HIP(hipMemsetAsync(p->vals[i], 0, n * 4u, st));
     }
-    MB_HIP(hipMemsetAsync(p->rows, 0, (3 * (size_t)mb::kMaxRows + 1) * 4u, st));
+    MB_HIP(hipMemsetAsync(p->counters, 0, kCounterWords * 4u, st));
     MB_HIP(hipStreamSynchronize(st));
     p->time = -1;
     return MB_OK;
@@ -368,7 +407,7 @@ static int run_frame(mb_physics *p, hipStream_t st)
     p->time += 1;
     const int par = (int)(p->time & 1);
     const int q = 1 - par;
-    std::array<hipEvent_t, 4> *ev = nullptr;
+    std::array<hipEvent_t, 3> *ev = nullptr;
     if (p->profiling) {
         constexpr size_t kRing = 4096;
         if (p->prof_used == kRing) {
@@ -376,27 +415,24 @@ static int run_frame(mb_physics *p, hipStream_t st)
             if (rc != MB_OK) return rc;
         }
         if (p->prof_used == p->prof_events.size()) {
-            std::array<hipEvent_t, 4> fe{};
+            std::array<hipEvent_t, 3> fe{};
             for (hipEvent_t &e : fe) MB_HIP(hipEventCreate(&e));
             p->prof_events.push_back(fe);
         }
         ev = &p->prof_events[p->prof_used++];
     }
     mb::FrameBuffers b;
-    b.keys_in = p->keys[par];
     b.keys_out = p->keys[q];
     b.perm = p->vals[q];
     b.off = p->off;
-    b.row_count = p->row_count();
-    b.row_fill = p->row_fill();
-    b.row_start = p->row_start();
-    b.pairs = p->pairs;
+    b.rb = p->buckets();
+    b.scratch_pairs = p->scratch_pairs;
     b.scratch_idx = p->scratch_idx;
     b.pos_i = p->pos[par];
     b.vfme_i = p->vfme[par];
     b.pos_o = p->pos[q];
     b.vfme_o = p->vfme[q];
-    hipError_t e = mb::launch_frame(b, p->consts, p->time == 0, p->n, st,
+    hipError_t e = mb::launch_frame(b, p->consts, p->time, p->n, st,
                                     ev ? ev->data() : nullptr);
     if (e != hipSuccess) return fail_hip(e, "frame");
     return MB_OK;
@@ -481,9 +517,9 @@ int mb_physics_profile_read(mb_physics *p, double *stage_ms, uint32_t capacity,
     DeviceGuard g(p->device);
     int rc = drain_profile(p);
     if (rc != MB_OK) return rc;
-    const size_t have = p->prof_ms.size() / 3;
+    const size_t have = p->prof_ms.size() / 2;
     const size_t take = have < capacity ? have : capacity;
-    if (stage_ms && take) std::memcpy(stage_ms, p->prof_ms.data(), take * 3 * sizeof(double));
+    if (stage_ms && take) std::memcpy(stage_ms, p->prof_ms.data(), take * 2 * sizeof(double));
     *frames = (uint32_t)take;
     p->prof_ms.clear();
     return MB_OK;

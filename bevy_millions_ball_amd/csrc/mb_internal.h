@@ -101,20 +101,47 @@ hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
 
 // ---- frame path ------------------------------------------------------------
 constexpr uint32_t kMaxRows = 4096;     // G <= 4096 (G^2 <= 2^24)
-constexpr uint32_t kPartItems = 4;
-constexpr uint32_t kPartTile = 256 * kPartItems;  // keys per partition block
 constexpr uint32_t kRowCap = 4096;      // spheres of one row sorted in LDS
 constexpr uint32_t kStageCap = 384;     // slots per staged neighbour-row window
 
+// Row buckets.  The collide (or INIT) kernel of frame t files every sphere's
+// (column, slot) pair under the row of its next cell; the row sort of frame
+// t + 1 sorts each row's bucket.  The fill counters are double buffered by
+// frame parity: the row sort of frame t reads set t & 1 (and the prefix of
+// the fills of the rows before its own) while it clears set !(t & 1), which
+// the collide of frame t then fills.  Pairs beyond a bucket's capacity spill
+// into one list (row << 42 | column << 30 | slot).
+// Every counter has a 128-byte cache line of its own (kLine words apart):
+// the atomics of neighbouring rows then do not queue on one line.
+constexpr uint32_t kLine = 32;
+
+struct RowBuckets {
+    uint64_t *bucket;         // G x cap pairs (column << 32 | slot)
+    uint64_t *spill;          // n entries
+    uint32_t *fill;           // 2 x G counters, set-major, kLine words apart
+    uint32_t *spill_count;    // 2 counters, kLine words apart
+    uint32_t cap;             // pairs per bucket
+};
+
+// Words of the counters of a G-row bucket set pair (both sets, both spill counts).
+inline size_t bucket_counter_words(uint32_t g) { return (2 * (size_t)g + 2) * kLine; }
+
+// Bucket capacity for n spheres over G rows: four times the mean row, at
+// least 512, a multiple of 64.
+inline uint32_t bucket_cap(uint32_t n, uint32_t g)
+{
+    const uint64_t mean = ((uint64_t)n + g - 1) / g;
+    uint64_t cap = 4 * mean < 512 ? 512 : 4 * mean;
+    cap = (cap + 63) & ~63ull;
+    return (uint32_t)cap;
+}
+
 struct FrameBuffers {
-    const uint32_t *keys_in;   // cell keys of the input state (unsorted)
-    uint32_t *keys_out;        // cell keys of the output state
+    uint32_t *keys_out;        // the sorted keys, then the keys of the next frame
     uint32_t *perm;            // sorted slot -> index into the input state
     uint32_t *off;             // G^2 + 1 per-cell slot offsets
-    uint32_t *row_count;       // kMaxRows: spheres per row of keys_in (zero on entry)
-    uint32_t *row_fill;        // kMaxRows: partition cursors
-    uint32_t *row_start;       // kMaxRows + 1: exclusive scan of row_count
-    uint64_t *pairs;           // n: (cx, index) partitioned by row
+    RowBuckets rb;
+    uint64_t *scratch_pairs;   // n: rows whose pairs spilled
     uint32_t *scratch_idx;     // n: rows larger than kRowCap
     const float4 *pos_i;
     const uint4 *vfme_i;
@@ -122,17 +149,17 @@ struct FrameBuffers {
     uint4 *vfme_o;
 };
 
-// Enqueues one frame: INIT (offsets fill + copy kernel) or LOOP (row count,
-// row partition, row sort, collide + integrate).  row_count and row_fill must
-// be zero on entry; the row sort leaves them zero.
-// `events` (4 entries or null) are recorded before the row count, before the
-// row sort, before collide + integrate and after it.
-hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, bool init, uint32_t n,
+// Enqueues frame `time`: INIT at time 0 (offsets, copy, keys, buckets for
+// frame 1), else LOOP (row sort of bucket set time & 1, collide + integrate
+// filing set !(time & 1)).  `events` (3 entries or null) are recorded before
+// the row sort, before collide + integrate and after it.
+hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time, uint32_t n,
                         hipStream_t stream, hipEvent_t *events);
 
-// key_out[i] = cell key of pos[i].
-hipError_t launch_rekey(const PhysConsts &c, const float4 *pos, uint32_t *key_out, uint32_t n,
-                        hipStream_t stream);
+// Zeroes bucket set `set`, then writes key_out[i] = cell key of pos[i] and
+// files every sphere (slot i) in that set.
+hipError_t launch_rekey(const PhysConsts &c, const float4 *pos, uint32_t *key_out,
+                        const RowBuckets &rb, uint32_t set, uint32_t n, hipStream_t stream);
 
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t count, hipStream_t stream);
 
@@ -147,11 +174,11 @@ struct SceneDesc {
     uint32_t pad_;
     float4 *pos[2];        // state ping-pong: frame t reads [t & 1], writes [!(t & 1)]
     uint4 *vfme[2];
-    uint32_t *keys;        // input keys, then the sorted keys, then the next keys (in place)
+    uint32_t *keys;        // the sorted keys, then the next keys (in place)
     uint32_t *perm;        // sorted slot -> index into the input state
     uint32_t *off;         // G^2 + 1 per-cell slot offsets
-    uint32_t *rows;        // row_count [G] | row_fill [G] | row_start [G + 1]; the INIT frame zeroes the counters
-    uint64_t *pairs;       // n
+    RowBuckets rb;         // the INIT frame clears the counters it fills
+    uint64_t *scratch_pairs; // n
     uint32_t *scratch_idx; // n
 };
 
@@ -160,8 +187,6 @@ struct SceneDesc {
 struct BatchPlan {
     SceneDesc *scenes;
     uint32_t scene_count;
-    const uint2 *tile_map;    // tiles of kPartTile input keys
-    uint32_t tiles;
     const uint2 *row_map;     // rows of the grid
     uint32_t rows;
     const uint2 *blk_plain;   // blocks of 256 sorted slots, spring off

@@ -806,25 +806,24 @@ hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
 }
 
 // ===========================================================================
-// The frame path: row partition -> per-row sort -> staged collide + integrate
+// The frame path: row buckets -> per-row sort -> staged collide + integrate
 // ===========================================================================
 //
 // A cell key is cz * G + cx, so the stable sort of the keys is the stable
 // partition of the spheres by row cz followed, inside each row, by the order
-// (cx, input index).  The frame path exploits that:
-//   row_count_kernel      row histogram of the input keys;
-//   row_partition_kernel  scatters (cx, index) pairs into their row's slot
-//                         range; each block reserves one sub-range per row it
-//                         touches (order inside a row is arbitrary);
-//   row_sort_kernel       one block per row: counting sort by cx in LDS, then
-//                         every pair is ranked among the pairs of its cell by
-//                         index, which restores argsort_stable exactly; it
-//                         writes perm, the sorted keys and the per-cell slot
-//                         offsets;
+// (cx, input index).  The frame path exploits that with two kernels:
+//   row_sort_kernel       one block per row: reads the row's bucket of
+//                         (cx, index) pairs (filed by the previous frame),
+//                         counting sort by cx in LDS, then every pair is
+//                         ranked among the pairs of its cell by index, which
+//                         restores argsort_stable exactly; it writes perm,
+//                         the sorted keys and the per-cell slot offsets;
 //   physics_step_kernel   collide + integrate with the neighbour-row windows
-//                         gathered into LDS; it writes the next keys.
-// Global atomics are limited to one per (block, row touched): many waves
-// adding to one counter serialise badly on this part.
+//                         gathered into LDS; it writes the next keys and
+//                         files every sphere in the bucket of its next row.
+// The input state of a frame is the previous frame's output in sorted order,
+// so the bucket of a sphere's next row is known the moment its step ends and
+// no separate counting or partition pass is needed.
 // No stage needs a cross-block ordering: there is no look-back and no spin.
 
 // Physical block b runs on XCD b % 8 (round-robin dispatch); give each XCD a
@@ -835,29 +834,6 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
     const uint32_t q = nb / kXcds, rem = nb % kXcds;
     const uint32_t x = b % kXcds, j = b / kXcds;
     return x * q + min(x, rem) + j;
-}
-
-// LDS counter[key] += lanes sharing `key`, one atomic per distinct key of
-// the wave; returns this lane's slot (old value + rank among those lanes).
-// Lanes with active == false take no slot.
-__device__ __forceinline__ uint32_t wave_slot_lds(uint32_t *counter, uint32_t key, bool active)
-{
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t lt = lanemask_lt(lane);
-    uint64_t todo = __ballot(active);
-    uint32_t slot = 0;
-    while (todo) {
-        const uint32_t leader = (uint32_t)__ffsll((long long)todo) - 1u;
-        const uint32_t lk = (uint32_t)__shfl((int)key, (int)leader, 64);
-        const bool mine = active && key == lk;
-        const uint64_t peers = __ballot(mine);
-        uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&counter[lk], (uint32_t)__popcll(peers));
-        base = (uint32_t)__shfl((int)base, (int)leader, 64);
-        if (mine) slot = base + (uint32_t)__popcll(peers & lt);
-        todo &= ~peers;
-    }
-    return slot;
 }
 
 // Exclusive scan, in place, of `count` (<= 4096) LDS words with a 256-thread
@@ -909,149 +885,52 @@ __device__ __forceinline__ uint32_t block256_scan_lds(uint32_t *vals, uint32_t c
     return total;
 }
 
-// Smallest and largest row among the block's keys (block-wide, via LDS).
-__device__ __forceinline__ void block_row_span(uint32_t lo, uint32_t hi, uint32_t *s_lo,
-                                               uint32_t *s_hi)
+// Files (column, slot) under `row` in a bucket set: the lanes of a wave that
+// share a row take consecutive places with one returning atomic per distinct
+// row, the leaders of all rows issuing theirs at once.  A pair beyond the
+// bucket's capacity goes to the spill list.  Every active lane files one pair.
+__device__ __forceinline__ void bucket_file(const RowBuckets &rb, uint32_t *__restrict__ fill,
+                                            uint32_t *__restrict__ spill_count, uint32_t row,
+                                            uint32_t col, uint32_t slot)
 {
     const uint32_t lane = threadIdx.x & 63u;
-#pragma unroll
-    for (uint32_t off = 32; off > 0; off >>= 1) {
-        lo = min(lo, (uint32_t)__shfl_xor((int)lo, (int)off, 64));
-        hi = max(hi, (uint32_t)__shfl_xor((int)hi, (int)off, 64));
-    }
-    if (threadIdx.x == 0) {
-        *s_lo = 0xFFFFFFFFu;
-        *s_hi = 0u;
-    }
-    __syncthreads();
-    if (lane == 0) {
-        atomicMin(s_lo, lo);
-        atomicMax(s_hi, hi);
-    }
-    __syncthreads();
-}
-
-// Row histogram of the frame's input keys: each block counts its tile in LDS
-// and adds one atomic per row it touches (nearly sorted keys touch few rows,
-// so every row counter sees about as many atomics as blocks overlap it).
-__device__ __forceinline__ void row_count_body(const uint32_t *__restrict__ keys, uint32_t n,
-                                               uint32_t grids, FastDiv gdiv,
-                                               uint32_t *__restrict__ row_count, uint32_t tile)
-{
-    __shared__ uint32_t s_hist[kMaxRows];
-    __shared__ uint32_t s_lo, s_hi;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t base = tile * kPartTile;
-    uint32_t row[kPartItems];
-    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
-#pragma unroll
-    for (uint32_t j = 0; j < kPartItems; ++j) {
-        const uint32_t i = base + j * 256u + tid;
-        row[j] = i < n ? min(fast_div(keys[i], gdiv), grids - 1u) : kInvalid;
-        if (i < n) {
-            lo = min(lo, row[j]);
-            hi = max(hi, row[j]);
+    const uint64_t lt = lanemask_lt(lane);
+    uint64_t todo = __ballot(1);
+    uint32_t leader_lane = lane, rank = 0, count = 0;
+    while (todo) {
+        const uint32_t leader = (uint32_t)__ffsll((long long)todo) - 1u;
+        const uint32_t lr = (uint32_t)__shfl((int)row, (int)leader, 64);
+        const uint64_t peers = __ballot(row == lr);
+        if (row == lr) {
+            leader_lane = leader;
+            rank = (uint32_t)__popcll(peers & lt);
         }
+        if (lane == leader) count = (uint32_t)__popcll(peers);
+        todo &= ~peers;
     }
-    block_row_span(lo, hi, &s_lo, &s_hi);
-    const uint32_t rlo = s_lo, rhi = s_hi;
-    if (rlo > rhi) return;  // empty tile (uniform across the block)
-    for (uint32_t r = rlo + tid; r <= rhi; r += 256u) s_hist[r] = 0u;
-    __syncthreads();
-#pragma unroll
-    for (uint32_t j = 0; j < kPartItems; ++j)
-        (void)wave_slot_lds(s_hist, row[j], row[j] != kInvalid);
-    __syncthreads();
-    for (uint32_t r = rlo + tid; r <= rhi; r += 256u) {
-        const uint32_t c = s_hist[r];
-        if (c) atomicAdd(&row_count[r], c);
+    uint32_t first = 0;
+    if (count) first = atomicAdd(&fill[row * kLine], count);
+    const uint32_t at = (uint32_t)__shfl((int)first, (int)leader_lane, 64) + rank;
+    if (at < rb.cap) {
+        rb.bucket[(size_t)row * rb.cap + at] = ((uint64_t)col << 32) | slot;
+    } else {
+        const uint32_t o = atomicAdd(spill_count, 1u);
+        rb.spill[o] = ((uint64_t)row << 42) | ((uint64_t)col << 30) | slot;
     }
 }
 
-__global__ __launch_bounds__(256) void row_count_kernel(const uint32_t *__restrict__ keys,
-                                                        uint32_t n, uint32_t grids, FastDiv gdiv,
-                                                        uint32_t *__restrict__ row_count)
-{
-    row_count_body(keys, n, grids, gdiv, row_count, blockIdx.x);
-}
-
-__device__ __forceinline__ void row_partition_body(
-    const uint32_t *__restrict__ keys, uint32_t n, uint32_t grids, FastDiv gdiv,
-    const uint32_t *__restrict__ row_count, uint32_t *__restrict__ row_fill,
-    uint32_t *__restrict__ row_start_out, uint64_t *__restrict__ pairs, uint32_t tile)
-{
-    __shared__ __align__(16) uint32_t s_start[kMaxRows + 4];
-    __shared__ uint32_t s_mine[kMaxRows];  // this block's count, then its base, per row
-    __shared__ uint32_t s_scratch[4];
-    __shared__ uint32_t s_lo, s_hi;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t rows = grids;
-    for (uint32_t r = tid; r < rows; r += 256u) s_start[r] = row_count[r];
-    __syncthreads();
-    const uint32_t total = block256_scan_lds(s_start, rows, s_scratch);
-    if (tid == 0) s_start[rows] = total;
-    __syncthreads();
-    if (tile == 0)
-        for (uint32_t r = tid; r <= rows; r += 256u) row_start_out[r] = s_start[r];
-
-    const uint32_t base = tile * kPartTile;
-    uint32_t key[kPartItems];
-    uint32_t slot[kPartItems];
-    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
-#pragma unroll
-    for (uint32_t j = 0; j < kPartItems; ++j) {
-        const uint32_t i = base + j * 256u + tid;
-        key[j] = i < n ? keys[i] : 0u;
-        if (i < n) {
-            const uint32_t r = min(fast_div(key[j], gdiv), rows - 1u);
-            lo = min(lo, r);
-            hi = max(hi, r);
-        }
-    }
-    block_row_span(lo, hi, &s_lo, &s_hi);
-    const uint32_t rlo = s_lo, rhi = s_hi;
-    if (rlo > rhi) return;
-    for (uint32_t r = rlo + tid; r <= rhi; r += 256u) s_mine[r] = 0u;
-    __syncthreads();
-#pragma unroll
-    for (uint32_t j = 0; j < kPartItems; ++j) {
-        const uint32_t i = base + j * 256u + tid;
-        const uint32_t r = min(fast_div(key[j], gdiv), rows - 1u);
-        slot[j] = wave_slot_lds(s_mine, r, i < n);
-    }
-    __syncthreads();
-    // One reservation per row this block touches.
-    for (uint32_t r = rlo + tid; r <= rhi; r += 256u) {
-        const uint32_t c = s_mine[r];
-        if (c) s_mine[r] = s_start[r] + atomicAdd(&row_fill[r], c);
-    }
-    __syncthreads();
-#pragma unroll
-    for (uint32_t j = 0; j < kPartItems; ++j) {
-        const uint32_t i = base + j * 256u + tid;
-        if (i < n) {
-            const uint32_t r = min(fast_div(key[j], gdiv), rows - 1u);
-            const uint32_t cx = min(key[j] - r * grids, grids - 1u);
-            const uint32_t dest = s_mine[r] + slot[j];
-            if (dest < s_start[r + 1u]) pairs[dest] = ((uint64_t)cx << 32) | i;
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void row_partition_kernel(
-    const uint32_t *__restrict__ keys, uint32_t n, uint32_t grids, FastDiv gdiv,
-    const uint32_t *__restrict__ row_count, uint32_t *__restrict__ row_fill,
-    uint32_t *__restrict__ row_start_out, uint64_t *__restrict__ pairs)
-{
-    row_partition_body(keys, n, grids, gdiv, row_count, row_fill, row_start_out, pairs, blockIdx.x);
-}
-
-// One block per row.  Dynamic LDS: `grids` cell counters (padded to a
-// multiple of 1024 words for the vector scan) followed by kRowCap indices.
+// One block per row r of frame `time` (>= 1).  The row's pairs are in bucket
+// set time & 1 (spilled pairs in the spill list); its slots start after the
+// spheres of all lower rows.  Counting sort by column in LDS, then each index
+// is ranked among the indices of its cell, which makes the result exactly
+// the stable argsort of the keys.  Writes perm, the sorted keys and the
+// per-cell offsets of the row, and clears bucket set !(time & 1) for the
+// collide kernel that follows.  Dynamic LDS: `grids` cell counters (padded
+// to a multiple of 1024 words for the vector scan) followed by kRowCap
+// indices.
 __device__ __forceinline__ void row_sort_body(
-    const uint64_t *__restrict__ pairs, uint32_t *__restrict__ scratch_idx,
-    const uint32_t *__restrict__ row_start, uint32_t *__restrict__ row_count,
-    uint32_t *__restrict__ row_fill, uint32_t grids, uint32_t n,
+    const RowBuckets &rb, uint32_t set, uint64_t *__restrict__ scratch_pairs,
+    uint32_t *__restrict__ scratch_idx, uint32_t grids, uint32_t n,
     uint32_t *__restrict__ perm, uint32_t *__restrict__ off, uint32_t *__restrict__ sorted_keys,
     uint32_t r)
 {
@@ -1059,10 +938,37 @@ __device__ __forceinline__ void row_sort_body(
     uint32_t *s_cell = s_dyn;                              // grids words (padded)
     uint32_t *s_idx = s_dyn + ((grids + 1023u) & ~1023u);  // kRowCap words
     __shared__ uint32_t s_scratch[4];
+    __shared__ uint32_t s_spilled;
     const uint32_t tid = threadIdx.x;
-    const uint32_t base = row_start[r];
-    const uint32_t m = row_start[r + 1u] - base;
+    const uint32_t *fill_in = rb.fill + set * grids * kLine;
+    uint32_t part = 0;
+    for (uint32_t q = tid; q < r; q += 256u) part += fill_in[q * kLine];
+    uint32_t base;
+    (void)block256_exclusive_scan(part, s_scratch, base);  // base = spheres in rows < r
+    const uint32_t m = fill_in[r * kLine];
+    if (tid == 0) {
+        rb.fill[((set ^ 1u) * grids + r) * kLine] = 0u;
+        if (r == 0) rb.spill_count[(set ^ 1u) * kLine] = 0u;
+        s_spilled = 0u;
+    }
     for (uint32_t c = tid; c < grids; c += 256u) s_cell[c] = 0u;
+    const uint64_t *src = rb.bucket + (size_t)r * rb.cap;
+    if (m > rb.cap) {
+        // the bucket and the row's spilled pairs, gathered in the row's slots
+        // of the scratch array
+        for (uint32_t e = tid; e < rb.cap; e += 256u) scratch_pairs[base + e] = src[e];
+        __syncthreads();
+        const uint32_t spilled = rb.spill_count[set * kLine];
+        for (uint32_t i = tid; i < spilled; i += 256u) {
+            const uint64_t v = rb.spill[i];
+            if ((uint32_t)(v >> 42) == r) {
+                const uint32_t at = rb.cap + atomicAdd(&s_spilled, 1u);
+                scratch_pairs[base + at] = (((v >> 30) & 0xFFFull) << 32) | (v & 0x3FFFFFFFull);
+            }
+        }
+        __threadfence_block();
+        src = scratch_pairs + base;
+    }
     __syncthreads();
 
     if (m <= kRowCap) {
@@ -1071,7 +977,7 @@ __device__ __forceinline__ void row_sort_body(
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) {
             const uint32_t e = j * 256u + tid;
-            el[j] = e < m ? pairs[base + e] : 0ull;
+            el[j] = e < m ? src[e] : 0ull;
         }
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) {
@@ -1120,19 +1026,19 @@ __device__ __forceinline__ void row_sort_body(
         // A row larger than the LDS tile: the same steps with the scattered
         // indices in global scratch.
         for (uint32_t e = tid; e < m; e += 256u)
-            atomicAdd(&s_cell[(uint32_t)(pairs[base + e] >> 32)], 1u);
+            atomicAdd(&s_cell[(uint32_t)(src[e] >> 32)], 1u);
         __syncthreads();
         (void)block256_scan_lds(s_cell, grids, s_scratch);
         for (uint32_t c = tid; c < grids; c += 256u) off[r * grids + c] = base + s_cell[c];
         __syncthreads();
         for (uint32_t e = tid; e < m; e += 256u) {
-            const uint64_t v = pairs[base + e];
+            const uint64_t v = src[e];
             scratch_idx[base + atomicAdd(&s_cell[(uint32_t)(v >> 32)], 1u)] = (uint32_t)v;
         }
         __threadfence_block();
         __syncthreads();
         for (uint32_t e = tid; e < m; e += 256u) {
-            const uint64_t v = pairs[base + e];
+            const uint64_t v = src[e];
             const uint32_t c = (uint32_t)(v >> 32);
             const uint32_t idx = (uint32_t)v;
             const uint32_t cs = c ? s_cell[c - 1u] : 0u;
@@ -1143,21 +1049,16 @@ __device__ __forceinline__ void row_sort_body(
             sorted_keys[base + cs + rank] = r * grids + c;
         }
     }
-    if (tid == 0) {
-        row_count[r] = 0u;
-        row_fill[r] = 0u;
-        if (r == grids - 1u) off[grids * grids] = n;
-    }
+    if (tid == 0 && r == grids - 1u) off[grids * grids] = n;
 }
 
 __global__ __launch_bounds__(256) void row_sort_kernel(
-    const uint64_t *__restrict__ pairs, uint32_t *__restrict__ scratch_idx,
-    const uint32_t *__restrict__ row_start, uint32_t *__restrict__ row_count,
-    uint32_t *__restrict__ row_fill, uint32_t grids, uint32_t n,
+    RowBuckets rb, uint32_t set, uint64_t *__restrict__ scratch_pairs,
+    uint32_t *__restrict__ scratch_idx, uint32_t grids, uint32_t n,
     uint32_t *__restrict__ perm, uint32_t *__restrict__ off, uint32_t *__restrict__ sorted_keys)
 {
-    row_sort_body(pairs, scratch_idx, row_start, row_count, row_fill, grids, n, perm, off,
-                  sorted_keys, xcd_block(blockIdx.x, gridDim.x));
+    row_sort_body(rb, set, scratch_pairs, scratch_idx, grids, n, perm, off, sorted_keys,
+                  xcd_block(blockIdx.x, gridDim.x));
 }
 
 // solve_sphere_constraint behind a cheap conservative filter:
@@ -1304,7 +1205,8 @@ __device__ __forceinline__ void physics_step_body(
     const PhysConsts &c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
     uint32_t *__restrict__ keys, const float4 *__restrict__ pos_i,
     const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
-    uint32_t n, uint32_t blk, float4 *__restrict__ s_pos, uint2 *__restrict__ s_vm)
+    const RowBuckets &rb, uint32_t set_out, uint32_t n, uint32_t blk,
+    float4 *__restrict__ s_pos, uint2 *__restrict__ s_vm)
 {
     const uint32_t tid = threadIdx.x;
     const uint32_t s0 = blk * 256u;
@@ -1374,7 +1276,10 @@ __device__ __forceinline__ void physics_step_body(
     else
         walk_candidates<SPRING, false>(c, a, s, rg, s_pos, s_vm, perm, pos_i, vfme_i);
     jitter(a, s, ox, oy, oz);
-    (void)finish_agent<SPRING>(c, a, s, pos_o, vfme_o, keys);
+    const uint32_t next = finish_agent<SPRING>(c, a, s, pos_o, vfme_o, keys);
+    const uint32_t nz = fast_div(next, c.grids_div);
+    bucket_file(rb, rb.fill + set_out * G * kLine, rb.spill_count + set_out * kLine, nz,
+                next - nz * G, s);
 }
 
 template <bool SPRING>
@@ -1382,11 +1287,11 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     PhysConsts c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
     uint32_t *__restrict__ keys, const float4 *__restrict__ pos_i,
     const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
-    uint32_t n)
+    RowBuckets rb, uint32_t set_out, uint32_t n)
 {
     __shared__ float4 s_pos[3 * kStageCap];  // the three windows back to back
     __shared__ uint2 s_vm[3 * kStageCap];    // their velocity and mass halves
-    physics_step_body<SPRING>(c, perm, off, keys, pos_i, vfme_i, pos_o, vfme_o, n,
+    physics_step_body<SPRING>(c, perm, off, keys, pos_i, vfme_i, pos_o, vfme_o, rb, set_out, n,
                               xcd_block(blockIdx.x, gridDim.x), s_pos, s_vm);
 }
 
@@ -1396,7 +1301,7 @@ __device__ __forceinline__ void physics_init_body(
     const PhysConsts &c, uint32_t *__restrict__ perm, uint32_t *__restrict__ key_out,
     const float4 *__restrict__ pos_i,
     const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
-    uint32_t n, uint32_t s)
+    const RowBuckets &rb, uint32_t n, uint32_t s)
 {
     if (s >= n) return;
     perm[s] = s;
@@ -1408,28 +1313,43 @@ __device__ __forceinline__ void physics_init_body(
     vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
                            pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
     key_out[s] = cz * c.grids + cx;
+    bucket_file(rb, rb.fill + c.grids * kLine, rb.spill_count + kLine, cz, cx, s);  // set 1: frame 1
 }
 
 __global__ __launch_bounds__(256) void physics_init_step_kernel(
     PhysConsts c, uint32_t *__restrict__ perm, uint32_t *__restrict__ key_out,
     const float4 *__restrict__ pos_i,
     const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
-    uint32_t n)
+    RowBuckets rb, uint32_t n)
 {
-    physics_init_body(c, perm, key_out, pos_i, vfme_i, pos_o, vfme_o, n,
+    physics_init_body(c, perm, key_out, pos_i, vfme_i, pos_o, vfme_o, rb, n,
                       blockIdx.x * 256u + threadIdx.x);
 }
 
 // Offsets of the INIT frame: cell 0 holds every sphere.
+// Also clears bucket set 1, which the INIT step fills for frame 1.
 __global__ __launch_bounds__(256) void init_offsets_kernel(uint32_t *__restrict__ off, uint32_t cells,
-                                                           uint32_t n)
+                                                           uint32_t n, RowBuckets rb, uint32_t grids)
 {
     const uint32_t c = blockIdx.x * 256u + threadIdx.x;
     if (c <= cells) off[c] = c == 0 ? 0u : n;
+    if (c < grids) rb.fill[(grids + c) * kLine] = 0u;
+    if (c == 0) rb.spill_count[kLine] = 0u;
+}
+
+// Clears bucket set `set` (first kernel) and files every sphere in it with
+// its key (second kernel).
+__global__ __launch_bounds__(256) void bucket_clear_kernel(RowBuckets rb, uint32_t set,
+                                                           uint32_t grids)
+{
+    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    if (r < grids) rb.fill[(set * grids + r) * kLine] = 0u;
+    if (r == 0) rb.spill_count[set * kLine] = 0u;
 }
 
 __global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *__restrict__ pos,
-                                                    uint32_t *__restrict__ key_out, uint32_t n)
+                                                    uint32_t *__restrict__ key_out,
+                                                    RowBuckets rb, uint32_t set, uint32_t n)
 {
     const uint32_t s = blockIdx.x * 256u + threadIdx.x;
     if (s >= n) return;
@@ -1437,48 +1357,44 @@ __global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *
     uint32_t cx, cz;
     cell_of(c, p.x, p.z, cx, cz);
     key_out[s] = cz * c.grids + cx;
+    bucket_file(rb, rb.fill + set * c.grids * kLine, rb.spill_count + set * kLine, cz, cx, s);
 }
 
-hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, bool init, uint32_t n,
+hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time, uint32_t n,
                         hipStream_t stream, hipEvent_t *events)
 {
     if (n == 0) return hipSuccess;
     const uint32_t G = c.grids;
     const uint32_t cells = G * G;
     const dim3 grid((n + 255u) / 256u);
-    const dim3 tiles((n + kPartTile - 1u) / kPartTile);
     auto mark = [&](int i) {
         return events ? hipEventRecord(events[i], stream) : hipSuccess;
     };
     hipError_t e = mark(0);
     if (e != hipSuccess) return e;
-    if (init) {
-        hipLaunchKernelGGL(init_offsets_kernel, dim3((cells + 256u) / 256u), dim3(256), 0, stream,
-                           b.off, cells, n);
-        if ((e = mark(1)) != hipSuccess || (e = mark(2)) != hipSuccess) return e;
+    if (time == 0) {
+        hipLaunchKernelGGL(init_offsets_kernel, dim3((std::max(cells, G) + 256u) / 256u), dim3(256),
+                           0, stream, b.off, cells, n, b.rb, G);
+        if ((e = mark(1)) != hipSuccess) return e;
         hipLaunchKernelGGL(physics_init_step_kernel, grid, dim3(256), 0, stream, c, b.perm,
-                           b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, n);
-        if ((e = mark(3)) != hipSuccess) return e;
+                           b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, b.rb, n);
+        if ((e = mark(2)) != hipSuccess) return e;
         return hipGetLastError();
     }
-    hipLaunchKernelGGL(row_count_kernel, tiles, dim3(256), 0, stream, b.keys_in, n, G,
-                       c.grids_div, b.row_count);
-    hipLaunchKernelGGL(row_partition_kernel, tiles, dim3(256), 0, stream, b.keys_in, n, G,
-                       c.grids_div, b.row_count, b.row_fill, b.row_start, b.pairs);
-    if ((e = mark(1)) != hipSuccess) return e;
+    const uint32_t set = (uint32_t)(time & 1);
     const size_t sort_lds = ((size_t)((G + 1023u) & ~1023u) + kRowCap) * 4u;
-    hipLaunchKernelGGL(row_sort_kernel, dim3(G), dim3(256), sort_lds, stream, b.pairs,
-                       b.scratch_idx, b.row_start, b.row_count, b.row_fill, G, n, b.perm, b.off,
-                       b.keys_out);
-    if ((e = mark(2)) != hipSuccess) return e;
+    hipLaunchKernelGGL(row_sort_kernel, dim3(G), dim3(256), sort_lds, stream, b.rb, set,
+                       b.scratch_pairs, b.scratch_idx, G, n, b.perm, b.off, b.keys_out);
+    if ((e = mark(1)) != hipSuccess) return e;
     if (c.enable_spring) {
         hipLaunchKernelGGL(physics_step_kernel<true>, grid, dim3(256), 0, stream, c, b.perm, b.off,
-                           b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, n);
+                           b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, b.rb, set ^ 1u, n);
     } else {
         hipLaunchKernelGGL(physics_step_kernel<false>, grid, dim3(256), 0, stream, c, b.perm,
-                           b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, n);
+                           b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, b.rb, set ^ 1u,
+                           n);
     }
-    if ((e = mark(3)) != hipSuccess) return e;
+    if ((e = mark(2)) != hipSuccess) return e;
     return hipGetLastError();
 }
 
@@ -1486,8 +1402,8 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, bool init, u
 // Each block looks up (scene, index within the scene) in its kernel's work
 // list and runs the single-scene body on that scene's arrays, so a scene's
 // results are the same alone and in any mix.  A scene at time 0 runs the
-// INIT frame: the sort kernels skip it (its keys are not read), the row sort
-// blocks write its INIT offsets and the collide blocks copy its state.
+// INIT frame: its row sort blocks write its INIT offsets and clear its
+// bucket set 1, and its collide blocks copy its state and fill that set.
 
 // Pointers loaded from a SceneDesc are generic; the round trip through the
 // global address space lets the compiler use global (not flat) accesses.
@@ -1499,25 +1415,16 @@ __device__ __forceinline__ T *gptr(T *p)
     return (T *)g;
 }
 
-__global__ __launch_bounds__(256) void batch_row_count_kernel(const SceneDesc *__restrict__ scenes,
-                                                              const uint2 *__restrict__ map)
+// Pointers of a scene's bucket set, in the global address space.
+__device__ __forceinline__ RowBuckets scene_buckets(const SceneDesc &S)
 {
-    const uint2 m = map[blockIdx.x];
-    const SceneDesc &S = scenes[m.x];
-    if (S.frames_left == 0u || S.time == 0) return;
-    row_count_body(gptr(S.keys), S.n, S.c.grids, S.c.grids_div, gptr(S.rows), m.y);
-}
-
-__global__ __launch_bounds__(256) void batch_row_partition_kernel(
-    const SceneDesc *__restrict__ scenes, const uint2 *__restrict__ map)
-{
-    const uint2 m = map[blockIdx.x];
-    const SceneDesc &S = scenes[m.x];
-    if (S.frames_left == 0u || S.time == 0) return;
-    const uint32_t G = S.c.grids;
-    uint32_t *rows = gptr(S.rows);
-    row_partition_body(gptr(S.keys), S.n, G, S.c.grids_div, rows, rows + G, rows + 2u * G,
-                       gptr(S.pairs), m.y);
+    RowBuckets rb;
+    rb.bucket = gptr(S.rb.bucket);
+    rb.spill = gptr(S.rb.spill);
+    rb.fill = gptr(S.rb.fill);
+    rb.spill_count = gptr(S.rb.spill_count);
+    rb.cap = S.rb.cap;
+    return rb;
 }
 
 __global__ __launch_bounds__(256) void batch_row_sort_kernel(const SceneDesc *__restrict__ scenes,
@@ -1528,20 +1435,19 @@ __global__ __launch_bounds__(256) void batch_row_sort_kernel(const SceneDesc *__
     if (S.frames_left == 0u) return;
     const uint32_t G = S.c.grids;
     uint32_t *off = gptr(S.off);
-    uint32_t *rows = gptr(S.rows);
+    const RowBuckets rb = scene_buckets(S);
     if (S.time == 0) {
-        // INIT offsets (cell 0 holds every sphere), and the row counters the
-        // first LOOP frame expects to find zero
+        // INIT: cell 0 holds every sphere; clear bucket set 1 for the INIT step
         for (uint32_t c = threadIdx.x; c < G; c += 256u)
             off[m.y * G + c] = (m.y == 0u && c == 0u) ? 0u : S.n;
         if (threadIdx.x == 0) {
             if (m.y == G - 1u) off[G * G] = S.n;
-            rows[m.y] = 0u;      // row_count
-            rows[G + m.y] = 0u;  // row_fill
+            rb.fill[(G + m.y) * kLine] = 0u;
+            if (m.y == 0u) rb.spill_count[kLine] = 0u;
         }
         return;
     }
-    row_sort_body(gptr(S.pairs), gptr(S.scratch_idx), rows + 2u * G, rows, rows + G, G, S.n,
+    row_sort_body(rb, (uint32_t)S.time & 1u, gptr(S.scratch_pairs), gptr(S.scratch_idx), G, S.n,
                   gptr(S.perm), off, gptr(S.keys), m.y);
 }
 
@@ -1554,15 +1460,16 @@ __global__ __launch_bounds__(256) void batch_physics_step_kernel(
     const uint2 m = map[xcd_block(blockIdx.x, gridDim.x)];
     const SceneDesc &S = scenes[m.x];
     if (S.frames_left == 0u) return;
+    const RowBuckets rb = scene_buckets(S);
     if (S.time == 0) {
         physics_init_body(S.c, gptr(S.perm), gptr(S.keys), gptr(S.pos[0]), gptr(S.vfme[0]),
-                          gptr(S.pos[1]), gptr(S.vfme[1]), S.n, m.y * 256u + threadIdx.x);
+                          gptr(S.pos[1]), gptr(S.vfme[1]), rb, S.n, m.y * 256u + threadIdx.x);
         return;
     }
     const uint32_t p = (uint32_t)S.time & 1u;
     physics_step_body<SPRING>(S.c, gptr(S.perm), gptr(S.off), gptr(S.keys), gptr(S.pos[p]),
-                              gptr(S.vfme[p]), gptr(S.pos[p ^ 1u]), gptr(S.vfme[p ^ 1u]), S.n,
-                              m.y, s_pos, s_vm);
+                              gptr(S.vfme[p]), gptr(S.pos[p ^ 1u]), gptr(S.vfme[p ^ 1u]), rb,
+                              p ^ 1u, S.n, m.y, s_pos, s_vm);
 }
 
 __global__ __launch_bounds__(256) void batch_advance_kernel(SceneDesc *__restrict__ scenes,
@@ -1598,12 +1505,6 @@ hipError_t launch_copy_words(uint32_t *dst, const uint32_t *src, uint32_t words,
 hipError_t launch_batch_frame(const BatchPlan &p, hipStream_t stream)
 {
     if (p.scene_count == 0) return hipSuccess;
-    if (p.tiles) {
-        hipLaunchKernelGGL(batch_row_count_kernel, dim3(p.tiles), dim3(256), 0, stream, p.scenes,
-                           p.tile_map);
-        hipLaunchKernelGGL(batch_row_partition_kernel, dim3(p.tiles), dim3(256), 0, stream,
-                           p.scenes, p.tile_map);
-    }
     if (p.rows) {
         const size_t sort_lds = ((size_t)((p.max_grids + 1023u) & ~1023u) + kRowCap) * 4u;
         hipLaunchKernelGGL(batch_row_sort_kernel, dim3(p.rows), dim3(256), sort_lds, stream,
@@ -1620,12 +1521,14 @@ hipError_t launch_batch_frame(const BatchPlan &p, hipStream_t stream)
     return hipGetLastError();
 }
 
-hipError_t launch_rekey(const PhysConsts &c, const float4 *pos, uint32_t *key_out, uint32_t n,
-                        hipStream_t stream)
+hipError_t launch_rekey(const PhysConsts &c, const float4 *pos, uint32_t *key_out,
+                        const RowBuckets &rb, uint32_t set, uint32_t n, hipStream_t stream)
 {
     if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(bucket_clear_kernel, dim3((c.grids + 255u) / 256u), dim3(256), 0, stream, rb,
+                       set, c.grids);
     hipLaunchKernelGGL(rekey_kernel, dim3((n + 255u) / 256u), dim3(256), 0, stream, c, pos,
-                       key_out, n);
+                       key_out, rb, set, n);
     return hipGetLastError();
 }
 

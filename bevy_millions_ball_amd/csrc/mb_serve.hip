@@ -63,7 +63,9 @@ bool finite_half(uint32_t h16) { return ((h16 >> 10) & 0x1Fu) != 0x1Fu; }
 
 // Byte offsets of one scene's arrays inside its region.
 struct SceneLayout {
-    uint64_t pos[2], vfme[2], keys, perm, scratch, pairs, off, rows, bytes;
+    uint64_t pos[2], vfme[2], keys, perm, scratch, scratch_pairs, spill, bucket, off, counters,
+        bytes;
+    uint32_t cap;
 };
 
 SceneLayout scene_layout(uint64_t n, uint64_t g)
@@ -82,9 +84,12 @@ SceneLayout scene_layout(uint64_t n, uint64_t g)
     l.keys = take(n * 4);
     l.perm = take(n * 4);
     l.scratch = take(n * 4);
-    l.pairs = take(n * 8);
+    l.scratch_pairs = take(n * 8);
+    l.spill = take(n * 8);
+    l.cap = mb::bucket_cap((uint32_t)n, (uint32_t)g);
+    l.bucket = take(g * l.cap * 8);
     l.off = take((g * g + 1) * 4);
-    l.rows = take((3 * g + 1) * 4);
+    l.counters = take(mb::bucket_counter_words((uint32_t)g) * 4);  // fill [2][G] | spill counts [2]
     l.bytes = at;
     return l;
 }
@@ -232,8 +237,12 @@ struct Request {
         d.keys = reinterpret_cast<uint32_t *>(region + layout.keys);
         d.perm = reinterpret_cast<uint32_t *>(region + layout.perm);
         d.off = reinterpret_cast<uint32_t *>(region + layout.off);
-        d.rows = reinterpret_cast<uint32_t *>(region + layout.rows);
-        d.pairs = reinterpret_cast<uint64_t *>(region + layout.pairs);
+        d.rb.bucket = reinterpret_cast<uint64_t *>(region + layout.bucket);
+        d.rb.spill = reinterpret_cast<uint64_t *>(region + layout.spill);
+        d.rb.fill = reinterpret_cast<uint32_t *>(region + layout.counters);
+        d.rb.spill_count = d.rb.fill + 2 * (size_t)g * mb::kLine;
+        d.rb.cap = layout.cap;
+        d.scratch_pairs = reinterpret_cast<uint64_t *>(region + layout.scratch_pairs);
         d.scratch_idx = reinterpret_cast<uint32_t *>(region + layout.scratch);
         return d;
     }
@@ -452,21 +461,19 @@ int upload_plan(mb_server *s)
 {
     const uint32_t count = (uint32_t)s->running.size();
     std::vector<mb::SceneDesc> descs;
-    std::vector<uint2> tiles, rows, plain, spring;
+    std::vector<uint2> rows, plain, spring;
     descs.reserve(count);
     uint32_t max_g = 1;
     for (uint32_t i = 0; i < count; ++i) {
         const Request *r = s->running[i];
         descs.push_back(r->desc());
         max_g = std::max(max_g, r->g);
-        for (uint32_t t = 0; t < (r->n + mb::kPartTile - 1) / mb::kPartTile; ++t)
-            tiles.push_back(make_uint2(i, t));
         for (uint32_t g = 0; g < r->g; ++g) rows.push_back(make_uint2(i, g));
         auto &blk = r->consts.enable_spring ? spring : plain;
         for (uint32_t b = 0; b < (r->n + 255u) / 256u; ++b) blk.push_back(make_uint2(i, b));
     }
     const size_t desc_bytes = align_up(descs.size() * sizeof(mb::SceneDesc));
-    const size_t map_count = tiles.size() + rows.size() + plain.size() + spring.size();
+    const size_t map_count = rows.size() + plain.size() + spring.size();
     const size_t need = desc_bytes + map_count * sizeof(uint2);
     if (s->plan_cap < need) {
         // grow geometrically; frames in flight may still read the old buffer
@@ -491,7 +498,7 @@ int upload_plan(mb_server *s)
     std::memcpy(st, descs.data(), descs.size() * sizeof(mb::SceneDesc));
     uint2 *m = reinterpret_cast<uint2 *>(st + desc_bytes);
     size_t at = 0;
-    for (auto *v : {&tiles, &rows, &plain, &spring}) {
+    for (auto *v : {&rows, &plain, &spring}) {
         std::memcpy(m + at, v->data(), v->size() * sizeof(uint2));
         at += v->size();
     }
@@ -506,9 +513,7 @@ int upload_plan(mb_server *s)
     mb::BatchPlan &p = s->plan;
     p.scenes = reinterpret_cast<mb::SceneDesc *>(s->d_plan);
     p.scene_count = count;
-    p.tile_map = reinterpret_cast<const uint2 *>(s->d_plan + desc_bytes);
-    p.tiles = (uint32_t)tiles.size();
-    p.row_map = p.tile_map + tiles.size();
+    p.row_map = reinterpret_cast<const uint2 *>(s->d_plan + desc_bytes);
     p.rows = (uint32_t)rows.size();
     p.blk_plain = p.row_map + rows.size();
     p.blk_plain_count = (uint32_t)plain.size();

@@ -245,9 +245,10 @@ class Physics:
         check(lib().mb_physics_set_profiling(self._h, int(bool(enable))))
 
     def profile_read(self, capacity: int = 1 << 16) -> np.ndarray:
-        """(frames, 3) device milliseconds of [sort, hashing, physics] per
+        """(frames, 2) device milliseconds of [sort (with the cell ranges),
+        collide + integrate (with the next frame's bucket filing)] per
         profiled frame since the last read."""
-        buf = np.empty((capacity, 3), np.float64)
+        buf = np.empty((capacity, 2), np.float64)
         f = C.c_uint32()
         check(lib().mb_physics_profile_read(self._h, _ptr(buf), capacity, C.byref(f)))
         return buf[: f.value].copy()

@@ -127,14 +127,14 @@ int mb_physics_debug_download(const mb_physics *physics, uint32_t *perm,
                               uint32_t *start, uint32_t *close,
                               uint32_t *next_keys, void *stream);
 
-/* Per-frame device time of the three stages of a frame, measured with HIP
+/* Per-frame device time of the two stages of a frame, measured with HIP
  * events recorded on the frame's stream while profiling is enabled:
- * [0] row partition (first half of the sort), [1] row sort (second half of
- * the sort, which also writes the cell ranges of spatial hashing),
- * [2] collide + integrate.  Recording does not block.
- * mb_physics_profile_read waits for the recorded frames, writes up to
- * `capacity` rows of three milliseconds values to stage_ms (may be NULL to
- * only count), stores the row count in *frames and forgets them. */
+ * [0] sort (the row sort, which also writes the cell ranges of spatial
+ * hashing), [1] collide + integrate (which also files every sphere in the
+ * row bucket of its next cell for the next frame's sort).  Recording does
+ * not block.  mb_physics_profile_read waits for the recorded frames, writes
+ * up to `capacity` rows of two milliseconds values to stage_ms (may be NULL
+ * to only count), stores the row count in *frames and forgets them. */
 int mb_physics_set_profiling(mb_physics *physics, int enable);
 int mb_physics_profile_read(mb_physics *physics, double *stage_ms, uint32_t capacity,
                             uint32_t *frames);

@@ -73,10 +73,13 @@ def test_poisson_requests_are_seeded_and_in_range():
 
 def test_scene_bytes_covers_every_array():
     n, g = 100_000, 512
-    per_sphere = 16 * 4 + 4 * 3 + 8  # pos x2, vfme x2, keys, perm, scratch, pairs
-    want = per_sphere * n + 4 * (g * g + 1) + 4 * (3 * g + 1)
+    # pos x2, vfme x2, keys, perm, scratch indices, scratch pairs, spill list
+    per_sphere = 16 * 4 + 4 * 3 + 8 + 8
+    cap = max(4 * -(-n // g), 512)
+    cap = -(-cap // 64) * 64
+    want = per_sphere * n + 8 * g * cap + 4 * (g * g + 1) + 4 * 32 * (2 * g + 2)
     got = Server.scene_bytes(n, g)
-    assert want <= got <= want + 10 * 256
+    assert want <= got <= want + 12 * 256
 
 
 def test_server_rejects_zero_cap():
