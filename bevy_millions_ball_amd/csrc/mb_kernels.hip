@@ -919,6 +919,71 @@ __device__ __forceinline__ void bucket_file(const RowBuckets &rb, uint32_t *__re
     }
 }
 
+// Filing for the collide kernel, per block.  A wave counts its spheres per
+// next row in LDS (rows within kFileWin of the block's first row; a sphere
+// further away is filed at once by bucket_file), records (row, rank, column)
+// and leaves; the last wave of the block to finish reserves each row's range
+// with one global atomic and writes all of the block's pairs.  No wave waits
+// for another, and only the last one waits for a global atomic.
+constexpr uint32_t kFileWin = 32;  // rows row0 - kFileLo .. row0 + kFileWin - 1 - kFileLo
+constexpr uint32_t kFileLo = 15;
+
+struct BlockFiling {
+    uint32_t count[kFileWin];  // spheres per window row (zeroed before the block's barrier)
+    uint32_t base[kFileWin];   // reserved start in each row's bucket
+    uint32_t entry[256];       // window row << 24 | rank << 12 | column, or kInvalid
+    uint32_t done;             // waves finished (zeroed before the block's barrier)
+};
+
+// Called by the active lanes of every wave that holds a valid slot (slots
+// s0 .. s0 + last); `row0` is the row of the block's first slot.
+__device__ __forceinline__ void block_file(BlockFiling &f, const RowBuckets &rb,
+                                           uint32_t *__restrict__ fill,
+                                           uint32_t *__restrict__ spill_count, uint32_t row0,
+                                           uint32_t row, uint32_t col, uint32_t s0, uint32_t last)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint32_t w = row + kFileLo - row0;  // u32: rows outside the window wrap high
+    uint32_t entry = kInvalid;
+    if (w < kFileWin) {
+        const uint32_t rank = atomicAdd(&f.count[w], 1u);
+        entry = (w << 24) | (rank << 12) | col;
+    } else {
+        bucket_file(rb, fill, spill_count, row, col, s0 + tid);
+    }
+    f.entry[tid] = entry;
+    __threadfence_block();
+    const uint64_t act = __ballot(1);
+    const uint32_t lane = tid & 63u;
+    const uint32_t first = (uint32_t)__ffsll((long long)act) - 1u;
+    uint32_t prev = 0;
+    if (lane == first) prev = atomicAdd(&f.done, 1u);
+    prev = (uint32_t)__shfl((int)prev, (int)first, 64);
+    if (prev != last / 64u) return;  // not the last of the (last / 64 + 1) waves
+    __threadfence_block();
+    const uint32_t li = (uint32_t)__popcll(act & lanemask_lt(lane));
+    const uint32_t nact = (uint32_t)__popcll(act);
+    for (uint32_t i = li; i < kFileWin; i += nact) {
+        const uint32_t cnt = f.count[i];
+        if (cnt) f.base[i] = atomicAdd(&fill[(row0 + i - kFileLo) * kLine], cnt);
+    }
+    __threadfence_block();
+    for (uint32_t i = li; i <= last; i += nact) {
+        const uint32_t e = f.entry[i];
+        if (e == kInvalid) continue;
+        const uint32_t wr = e >> 24;
+        const uint32_t at = f.base[wr] + ((e >> 12) & 0xFFFu);
+        const uint32_t r = row0 + wr - kFileLo;
+        const uint32_t cl = e & 0xFFFu;
+        if (at < rb.cap) {
+            rb.bucket[(size_t)r * rb.cap + at] = ((uint64_t)cl << 32) | (s0 + i);
+        } else {
+            const uint32_t o = atomicAdd(spill_count, 1u);
+            rb.spill[o] = ((uint64_t)r << 42) | ((uint64_t)cl << 30) | (s0 + i);
+        }
+    }
+}
+
 // One block per row r of frame `time` (>= 1).  The row's pairs are in bucket
 // set time & 1 (spilled pairs in the spill list); its slots start after the
 // spheres of all lower rows.  Counting sort by column in LDS, then each index
@@ -1208,6 +1273,7 @@ __device__ __forceinline__ void physics_step_body(
     const RowBuckets &rb, uint32_t set_out, uint32_t n, uint32_t blk,
     float4 *__restrict__ s_pos, uint2 *__restrict__ s_vm)
 {
+    __shared__ BlockFiling s_file;
     const uint32_t tid = threadIdx.x;
     const uint32_t s0 = blk * 256u;
     const uint32_t last = min(n - s0, 256u) - 1u;
@@ -1215,6 +1281,8 @@ __device__ __forceinline__ void physics_step_body(
     const bool valid = tid <= last;
     const uint32_t G = c.grids;
     const uint32_t C = G * G;
+    if (tid < kFileWin) s_file.count[tid] = 0u;
+    if (tid == 0) s_file.done = 0u;
 
     // `keys` holds the sorted keys on entry (the cell of each slot's sphere,
     // computed from the same position by the previous frame) and receives
@@ -1278,8 +1346,8 @@ __device__ __forceinline__ void physics_step_body(
     jitter(a, s, ox, oy, oz);
     const uint32_t next = finish_agent<SPRING>(c, a, s, pos_o, vfme_o, keys);
     const uint32_t nz = fast_div(next, c.grids_div);
-    bucket_file(rb, rb.fill + set_out * G * kLine, rb.spill_count + set_out * kLine, nz,
-                next - nz * G, s);
+    block_file(s_file, rb, rb.fill + set_out * G * kLine, rb.spill_count + set_out * kLine,
+               fast_div(kfirst, c.grids_div), nz, next - nz * G, s0, last);
 }
 
 template <bool SPRING>
