@@ -102,7 +102,7 @@ hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
 // ---- frame path ------------------------------------------------------------
 constexpr uint32_t kMaxRows = 4096;     // G <= 4096 (G^2 <= 2^24)
 constexpr uint32_t kRowCap = 4096;      // spheres of one row sorted in LDS
-constexpr uint32_t kStageCap = 384;     // slots per staged neighbour-row window
+constexpr uint32_t kStagePool = 896;    // staged slots of a block's three neighbour-row windows
 
 // Row buckets.  The collide (or INIT) kernel of frame t files every sphere's
 // (column, slot) pair under the row of its next cell; the row sort of frame

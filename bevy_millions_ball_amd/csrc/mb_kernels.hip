@@ -1176,7 +1176,8 @@ struct Ranges {
 
 __device__ __forceinline__ Ranges make_ranges(uint32_t lo0, uint32_t hi0, uint32_t lo1,
                                               uint32_t hi1, uint32_t lo2, uint32_t hi2,
-                                              uint32_t wlo0, uint32_t wlo1, uint32_t wlo2)
+                                              uint32_t wlo0, uint32_t wlo1, uint32_t wlo2,
+                                              uint32_t base1, uint32_t base2)
 {
     Ranges r;
     r.n0 = hi0 - lo0;
@@ -1187,8 +1188,8 @@ __device__ __forceinline__ Ranges make_ranges(uint32_t lo0, uint32_t hi0, uint32
     r.slot0 = b0;
     r.slot1 = b1 - b0;
     r.slot2 = b2 - b1;
-    // LDS index: w * kStageCap + (slot - wlo_w)
-    const uint32_t l0 = b0 - wlo0, l1 = kStageCap + b1 - wlo1, l2 = 2u * kStageCap + b2 - wlo2;
+    // LDS index: base_w + (slot - wlo_w), windows back to back from base_0 = 0
+    const uint32_t l0 = b0 - wlo0, l1 = base1 + b1 - wlo1, l2 = base2 + b2 - wlo2;
     r.lds0 = l0;
     r.lds1 = l1 - l0;
     r.lds2 = l2 - l1;
@@ -1322,21 +1323,24 @@ __device__ __forceinline__ void physics_step_body(
         lo2 = off[k + G - lft];
         hi2 = off[k + G + rgt];
     }
-    const bool staged = wcnt[0] <= kStageCap && wcnt[1] <= kStageCap && wcnt[2] <= kStageCap;
+    // the three windows share one pool, back to back
+    const uint32_t wbase[3] = {0u, wcnt[0], wcnt[0] + wcnt[1]};
+    const bool staged = wcnt[0] + wcnt[1] + wcnt[2] <= kStagePool;
     if (staged) {
 #pragma unroll
         for (uint32_t w = 0; w < 3; ++w) {
             for (uint32_t j = tid; j < wcnt[w]; j += 256u) {
                 const uint32_t bo = perm[wlo[w] + j];
-                s_pos[w * kStageCap + j] = pos_i[bo];
-                if (!SPRING) s_vm[w * kStageCap + j] = vel_mass_of(vfme_i[bo]);
+                s_pos[wbase[w] + j] = pos_i[bo];
+                if (!SPRING) s_vm[wbase[w] + j] = vel_mass_of(vfme_i[bo]);
             }
         }
     }
     __syncthreads();
     if (!valid) return;
 
-    const Ranges rg = make_ranges(lo0, hi0, lo1, hi1, lo2, hi2, wlo[0], wlo[1], wlo[2]);
+    const Ranges rg = make_ranges(lo0, hi0, lo1, hi1, lo2, hi2, wlo[0], wlo[1], wlo[2], wbase[1],
+                                  wbase[2]);
     verlet(c, a);
     const float ox = a.px, oy = a.py, oz = a.pz;
     if (staged)
@@ -1357,8 +1361,8 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
     RowBuckets rb, uint32_t set_out, uint32_t n)
 {
-    __shared__ float4 s_pos[3 * kStageCap];  // the three windows back to back
-    __shared__ uint2 s_vm[3 * kStageCap];    // their velocity and mass halves
+    __shared__ float4 s_pos[kStagePool];  // the three windows back to back
+    __shared__ uint2 s_vm[kStagePool];    // their velocity and mass halves
     physics_step_body<SPRING>(c, perm, off, keys, pos_i, vfme_i, pos_o, vfme_o, rb, set_out, n,
                               xcd_block(blockIdx.x, gridDim.x), s_pos, s_vm);
 }
@@ -1523,8 +1527,8 @@ template <bool SPRING>
 __global__ __launch_bounds__(256) void batch_physics_step_kernel(
     const SceneDesc *__restrict__ scenes, const uint2 *__restrict__ map)
 {
-    __shared__ float4 s_pos[3 * kStageCap];
-    __shared__ uint2 s_vm[3 * kStageCap];
+    __shared__ float4 s_pos[kStagePool];
+    __shared__ uint2 s_vm[kStagePool];
     const uint2 m = map[xcd_block(blockIdx.x, gridDim.x)];
     const SceneDesc &S = scenes[m.x];
     if (S.frames_left == 0u) return;
