@@ -287,6 +287,24 @@ def test_edge_cases_tiny_worlds(gpu, oracle):
             compare_frames(g, r, 8)
 
 
+def test_dense_rows_spill_and_fallbacks(gpu, oracle):
+    """Rows far above the mean: bucket spill, the global-memory row sort and
+    the collide path that reads neighbours from global memory."""
+    from scene_helpers import dense_row_scene
+
+    ps, pos, vf = dense_row_scene()
+    g = Physics(ps)
+    g.upload_state(pos, vf)
+    r = oracle.World(ps)
+    r.upload(pos, vf)
+    with g:
+        compare_frames(g, r, 5)
+        # the spill path was taken: rows still far above the bucket capacity
+        keys = r.debug()["next_keys"]
+        rows = np.bincount(keys // ps.number_of_grids_one_dimension, minlength=64)
+        assert rows.max() > 1024
+
+
 def test_out_of_box_upload_is_clamped_like_oracle(gpu, oracle):
     """The reference demo's own start lattice lies partly outside its box
     (main.rs:43-44 with mod.rs:173-196); cell ids are clamped to the grid."""

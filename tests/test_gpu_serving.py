@@ -11,6 +11,7 @@ import pytest
 from bevy_millions_ball_amd import Physics
 from bevy_millions_ball_amd.scenes import SceneConfig, generate
 from bevy_millions_ball_amd.serving import Server, scene_grid
+from scene_helpers import dense_row_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -35,6 +36,7 @@ MIX = [
     scene(1, 2, 6, grids=2),                # one sphere, G^2 > N
     scene(70001, 0, 7),                     # INIT only, G = 512
     scene(9999, 9, 8, grids=100),           # G not a power of two
+    dense_row_scene() + (3,),               # bucket spill, global row sort, unstaged collide
 ]
 
 
