@@ -12,6 +12,7 @@ from .physics import (  # noqa: F401
     PhysicsPlugin,
     PhysicsSettings,
     default_settings_from_library,
+    inspect_checkpoint,
     physics_run,
     radix_sort_run,
     radix_sort_workspace_size,

@@ -129,6 +129,11 @@ SIGNATURES = {
         C.c_int,
         [C.POINTER(MbSettings), C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, C.c_uint32, _P],
     ),
+    "mb_checkpoint_size": (C.c_size_t, [C.c_uint32]),
+    "mb_physics_save_checkpoint": (C.c_int, [_P, _P, C.c_size_t]),
+    "mb_physics_load_checkpoint": (C.c_int, [_P, _P, C.c_size_t]),
+    "mb_checkpoint_inspect": (
+        C.c_int, [_P, C.c_size_t, C.POINTER(MbSettings), C.POINTER(C.c_int64), _U32P]),
     "mb_server_scene_bytes": (C.c_uint64, [C.c_uint32, C.c_uint32]),
     "mb_server_create": (C.c_int, [C.POINTER(MbServerConfig), C.POINTER(_P)]),
     "mb_server_destroy": (C.c_int, [_P]),

@@ -402,6 +402,43 @@ int mb_physics_download_state(const mb_physics *p, float *pos, uint32_t *vfme, v
     return MB_OK;
 }
 
+// ---- checkpoints --------------------------------------------------------
+namespace {
+
+constexpr char kCkMagic[4] = {'M', 'B', 'C', 'K'};
+constexpr uint32_t kCkVersion = 1;
+constexpr size_t kCkHeader = 4 + 4 + sizeof(mb_settings) + 8 + 4;
+
+int parse_checkpoint(const void *buffer, size_t size, mb_settings *settings, int64_t *time,
+                     uint32_t *n)
+{
+    if (!buffer) return fail(MB_ERR_INVALID_ARGUMENT, "null checkpoint");
+    if (size < kCkHeader) return fail(MB_ERR_INVALID_ARGUMENT, "checkpoint shorter than its header");
+    const char *b = static_cast<const char *>(buffer);
+    if (std::memcmp(b, kCkMagic, 4) != 0) return fail(MB_ERR_INVALID_ARGUMENT, "not a checkpoint");
+    uint32_t version;
+    std::memcpy(&version, b + 4, 4);
+    if (version != kCkVersion)
+        return fail(MB_ERR_INVALID_ARGUMENT, "unsupported checkpoint version " + std::to_string(version));
+    mb_settings s;
+    int64_t t;
+    uint32_t count;
+    std::memcpy(&s, b + 8, sizeof(s));
+    std::memcpy(&t, b + 8 + sizeof(s), 8);
+    std::memcpy(&count, b + 16 + sizeof(s), 4);
+    if (count != s.max_number_of_agents)
+        return fail(MB_ERR_INVALID_ARGUMENT, "checkpoint sphere count disagrees with its settings");
+    if (t < -1) return fail(MB_ERR_INVALID_ARGUMENT, "checkpoint time before -1");
+    if (size != kCkHeader + (size_t)count * 32u)
+        return fail(MB_ERR_INVALID_ARGUMENT, "checkpoint size does not match its sphere count");
+    if (settings) *settings = s;
+    if (time) *time = t;
+    if (n) *n = count;
+    return MB_OK;
+}
+
+}  // namespace
+
 static int run_frame(mb_physics *p, hipStream_t st)
 {
     p->time += 1;
@@ -448,6 +485,69 @@ int mb_physics_step(mb_physics *p, uint32_t frames, void *stream)
         int rc = run_frame(p, st);
         if (rc != MB_OK) return rc;
     }
+    return MB_OK;
+}
+
+size_t mb_checkpoint_size(uint32_t n) { return kCkHeader + (size_t)n * 32u; }
+
+int mb_checkpoint_inspect(const void *buffer, size_t size, mb_settings *settings, int64_t *time,
+                          uint32_t *n)
+{
+    return parse_checkpoint(buffer, size, settings, time, n);
+}
+
+int mb_physics_save_checkpoint(const mb_physics *p, void *buffer, size_t size)
+{
+    if (!p || !buffer) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    if (size != mb_checkpoint_size(p->n))
+        return fail(MB_ERR_INVALID_ARGUMENT, "buffer size must be mb_checkpoint_size(N)");
+    char *b = static_cast<char *>(buffer);
+    std::memcpy(b, kCkMagic, 4);
+    std::memcpy(b + 4, &kCkVersion, 4);
+    std::memcpy(b + 8, &p->settings, sizeof(mb_settings));
+    std::memcpy(b + 8 + sizeof(mb_settings), &p->time, 8);
+    std::memcpy(b + 16 + sizeof(mb_settings), &p->n, 4);
+    float *pos = reinterpret_cast<float *>(b + kCkHeader);
+    uint32_t *vfme = reinterpret_cast<uint32_t *>(b + kCkHeader + (size_t)p->n * 16u);
+    return mb_physics_download_state(p, pos, vfme, nullptr);
+}
+
+int mb_physics_load_checkpoint(mb_physics *p, const void *buffer, size_t size)
+{
+    if (!p) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    mb_settings s;
+    int64_t t;
+    uint32_t n;
+    int rc = parse_checkpoint(buffer, size, &s, &t, &n);
+    if (rc != MB_OK) return rc;
+    if (n != p->n)
+        return fail(MB_ERR_INVALID_ARGUMENT, "checkpoint holds " + std::to_string(n) +
+                                                 " spheres, the handle " + std::to_string(p->n));
+    rc = validate_settings(s);
+    if (rc != MB_OK) return rc;
+    const char *b = static_cast<const char *>(buffer);
+    const float *pos = reinterpret_cast<const float *>(b + kCkHeader);
+    const uint32_t *vfme = reinterpret_cast<const uint32_t *>(b + kCkHeader + (size_t)n * 16u);
+    // settings first (buckets sized for the grid), then the state as uploaded
+    p->settings = s;
+    p->consts = mb::compile_consts(s);
+    rc = alloc_buckets(p);
+    if (rc != MB_OK) return rc;
+    rc = mb_physics_upload_state(p, pos, vfme, nullptr);  // state in EVE, time = -1
+    if (rc != MB_OK) return rc;
+    if (t < 0) return MB_OK;
+    DeviceGuard g(p->device);
+    // frame t wrote state[(t + 1) & 1]; put the state there and file it for frame t + 1
+    const int cur = (int)((t + 1) & 1);
+    if (cur != 0) {
+        MB_HIP(hipMemcpy(p->pos[cur], p->pos[0], (size_t)n * sizeof(float4), hipMemcpyDeviceToDevice));
+        MB_HIP(hipMemcpy(p->vfme[cur], p->vfme[0], (size_t)n * sizeof(uint4), hipMemcpyDeviceToDevice));
+    }
+    hipError_t e = mb::launch_rekey(p->consts, p->pos[cur], p->keys[cur], p->buckets(),
+                                    (uint32_t)((t + 1) & 1), n, nullptr);
+    if (e != hipSuccess) return fail_hip(e, "checkpoint rekey");
+    MB_HIP(hipStreamSynchronize(nullptr));
+    p->time = t;
     return MB_OK;
 }
 

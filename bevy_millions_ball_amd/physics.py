@@ -135,6 +135,16 @@ class PhysicsSettings:
         )
 
 
+def inspect_checkpoint(data: bytes) -> Tuple[PhysicsSettings, int, int]:
+    """(settings, time, number of spheres) of a checkpoint; needs no device."""
+    buf = C.create_string_buffer(bytes(data), len(data))
+    s = MbSettings()
+    t = C.c_int64()
+    n = C.c_uint32()
+    check(lib().mb_checkpoint_inspect(buf, len(data), C.byref(s), C.byref(t), C.byref(n)))
+    return PhysicsSettings.from_c(s), t.value, n.value
+
+
 def default_settings_from_library() -> PhysicsSettings:
     s = MbSettings()
     lib().mb_settings_default(C.byref(s))
@@ -216,6 +226,19 @@ class Physics:
         vf = np.empty((self.number_of_agents, 4), np.uint32)
         check(lib().mb_physics_download_state(self._h, _ptr(pos), _ptr(vf), _stream_ptr(stream)))
         return pos, vf
+
+    # -- checkpoints --------------------------------------------------------
+    def save_checkpoint(self) -> bytes:
+        """Settings, time and the last frame's state (see mb_physics_save_checkpoint)."""
+        size = lib().mb_checkpoint_size(self.number_of_agents)
+        buf = (C.c_char * size)()
+        check(lib().mb_physics_save_checkpoint(self._h, buf, size))
+        return bytes(buf)
+
+    def load_checkpoint(self, data: bytes) -> None:
+        buf = C.create_string_buffer(bytes(data), len(data))
+        check(lib().mb_physics_load_checkpoint(self._h, buf, len(data)))
+        self._settings = self.settings
 
     # -- stepping (PhysicsNode::update + run) ----------------------------
     def run(self, frames: int = 1, stream=None) -> None:

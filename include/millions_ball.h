@@ -104,6 +104,25 @@ int mb_physics_upload_state_device(mb_physics *physics,
 int mb_physics_download_state(const mb_physics *physics, float *position_with_scale,
                               uint32_t *packed_vfme, void *stream);
 
+/* Checkpoints (no counterpart in the reference, whose state lives only in
+ * its GPU buffers).  A checkpoint is a little-endian byte buffer: the magic
+ * "MBCK", format version 1 (u32), mb_settings (60 bytes), time (i64), N (u32),
+ * then the state the last frame wrote: N float4 position_with_scale and N
+ * uint4 packed_vfme.  Loading puts that state back as the output of frame
+ * `time` and files it for the next frame exactly as that frame would have,
+ * so stepping on from a loaded checkpoint gives the same results, bit for
+ * bit, as stepping on without it.  A checkpoint taken before the first frame
+ * (time -1) holds the uploaded state.  The debug views of the last frame
+ * (perm, start / close) are not part of a checkpoint. */
+size_t mb_checkpoint_size(uint32_t number_of_agents);
+int mb_physics_save_checkpoint(const mb_physics *physics, void *buffer, size_t size);
+/* The handle's max_number_of_agents must equal the checkpoint's N; its
+ * settings are replaced by the checkpoint's (which may change the grid). */
+int mb_physics_load_checkpoint(mb_physics *physics, const void *buffer, size_t size);
+/* Reads a checkpoint's header without a device: settings, time and N. */
+int mb_checkpoint_inspect(const void *buffer, size_t size, mb_settings *settings, int64_t *time,
+                          uint32_t *number_of_agents);
+
 /* Runs `frames` frames of PhysicsNode::run (mod.rs:846-912): radix sort ->
  * spatial hashing -> physics (INIT when time == 0, else LOOP).  Does nothing
  * while enable_simulation is 0 (mod.rs:780-782, 853-855). */

@@ -305,6 +305,48 @@ def test_dense_rows_spill_and_fallbacks(gpu, oracle):
         assert rows.max() > 1024
 
 
+@pytest.mark.parametrize("split", [0, 1, 6])
+def test_checkpoint_resume_is_bit_exact(gpu, split):
+    """Save after `split` frames (0: before INIT), resume in a fresh handle
+    and in the same handle after stepping elsewhere: both continue exactly
+    like the uninterrupted run."""
+    from bevy_millions_ball_amd import inspect_checkpoint
+
+    cfg = replace(CONFIGS["plain_65k"], number_of_agents=8192, grids=64)
+    ps = cfg.settings()
+    pos, vf = generate(cfg, seed=5)
+    with Physics(ps) as a:
+        a.upload_state(pos, vf)
+        a.run(split)
+        ck = a.save_checkpoint()
+        assert inspect_checkpoint(ck)[1:] == (split - 1, 8192)
+        a.run(7)
+        want = a.download_state(), a.debug()["next_keys"]
+        a.load_checkpoint(ck)  # rewind the same handle
+        assert a.time() == split - 1
+        a.run(7)
+        again = a.download_state(), a.debug()["next_keys"]
+    with Physics(ps) as b:
+        b.load_checkpoint(ck)
+        b.run(7)
+        got = b.download_state(), b.debug()["next_keys"]
+    for res in (again, got):
+        assert np.array_equal(res[0][0].view(np.uint32), want[0][0].view(np.uint32))
+        assert np.array_equal(res[0][1], want[0][1])
+        assert np.array_equal(res[1], want[1])
+
+
+def test_checkpoint_rejects_other_sizes(gpu):
+    cfg = replace(CONFIGS["plain_65k"], number_of_agents=4096, grids=64)
+    with Physics(cfg.settings()) as a:
+        a.upload_state(*generate(cfg, seed=1))
+        ck = a.save_checkpoint()
+    other = replace(cfg, number_of_agents=8192)
+    with Physics(other.settings()) as b:
+        with pytest.raises(Exception, match="4096"):
+            b.load_checkpoint(ck)
+
+
 def test_out_of_box_upload_is_clamped_like_oracle(gpu, oracle):
     """The reference demo's own start lattice lies partly outside its box
     (main.rs:43-44 with mod.rs:173-196); cell ids are clamped to the grid."""
