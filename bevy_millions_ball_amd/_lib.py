@@ -48,6 +48,10 @@ class MbBuffers(C.Structure):
     ]
 
 
+class MbTuning(C.Structure):
+    _fields_ = [("scan_margin", C.c_float), ("stage_pool", C.c_uint32)]
+
+
 class MbServerConfig(C.Structure):
     _fields_ = [
         ("memory_cap_bytes", C.c_uint64),
@@ -129,6 +133,10 @@ SIGNATURES = {
         C.c_int,
         [C.POINTER(MbSettings), C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, C.c_uint32, _P],
     ),
+    "mb_physics_get_tuning": (C.c_int, [_P, C.POINTER(MbTuning)]),
+    "mb_physics_set_tuning": (C.c_int, [_P, C.POINTER(MbTuning)]),
+    "mb_physics_autotune": (
+        C.c_int, [_P, C.c_uint32, C.POINTER(MbTuning), C.POINTER(C.c_double)]),
     "mb_checkpoint_size": (C.c_size_t, [C.c_uint32]),
     "mb_physics_save_checkpoint": (C.c_int, [_P, _P, C.c_size_t]),
     "mb_physics_load_checkpoint": (C.c_int, [_P, _P, C.c_size_t]),

@@ -104,6 +104,7 @@ mb::PhysConsts mb::compile_consts(const mb_settings &s)
     c.enable_spring = s.enable_spring ? 1 : 0;
     c.grid_size_div = mb::make_fastdiv(s.grid_size);
     c.grids_div = mb::make_fastdiv(s.number_of_grids_one_dimension);
+    c.scan_margin = 1.0f;
     return c;
 }
 
@@ -163,6 +164,9 @@ struct mb_physics {
     std::vector<std::array<hipEvent_t, 3>> prof_events;
     size_t prof_used = 0;
     std::vector<double> prof_ms;  // [frame][2] of frames already read back
+
+    mb::FrameTuning tuning;      // launch choices that do not change results
+    float scan_margin = 1.0f;
 
     mb::RowBuckets buckets() const
     {
@@ -338,6 +342,7 @@ int mb_physics_set_settings(mb_physics *p, const mb_settings *s)
                         s->grid_size != p->settings.grid_size;
     p->settings = *s;
     p->consts = mb::compile_consts(*s);
+    p->consts.scan_margin = p->scan_margin;
     if (regrid && p->time >= 0) {
         // The next frame sorts keys of the new grid (the reference would sort
         // the stale keys of the old grid once, reading stale `close` words).
@@ -470,7 +475,7 @@ static int run_frame(mb_physics *p, hipStream_t st)
     b.pos_o = p->pos[q];
     b.vfme_o = p->vfme[q];
     hipError_t e = mb::launch_frame(b, p->consts, p->time, p->n, st,
-                                    ev ? ev->data() : nullptr);
+                                    ev ? ev->data() : nullptr, p->tuning);
     if (e != hipSuccess) return fail_hip(e, "frame");
     return MB_OK;
 }
@@ -531,6 +536,7 @@ int mb_physics_load_checkpoint(mb_physics *p, const void *buffer, size_t size)
     // settings first (buckets sized for the grid), then the state as uploaded
     p->settings = s;
     p->consts = mb::compile_consts(s);
+    p->consts.scan_margin = p->scan_margin;
     rc = alloc_buckets(p);
     if (rc != MB_OK) return rc;
     rc = mb_physics_upload_state(p, pos, vfme, nullptr);  // state in EVE, time = -1
@@ -548,6 +554,84 @@ int mb_physics_load_checkpoint(mb_physics *p, const void *buffer, size_t size)
     if (e != hipSuccess) return fail_hip(e, "checkpoint rekey");
     MB_HIP(hipStreamSynchronize(nullptr));
     p->time = t;
+    return MB_OK;
+}
+
+int mb_physics_get_tuning(const mb_physics *p, mb_tuning *t)
+{
+    if (!p || !t) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    t->scan_margin = p->scan_margin;
+    t->stage_pool = p->tuning.big_pool ? mb::kStagePoolBig : mb::kStagePool;
+    return MB_OK;
+}
+
+int mb_physics_set_tuning(mb_physics *p, const mb_tuning *t)
+{
+    if (!p || !t) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    if (!(t->scan_margin > 0.0f && t->scan_margin <= 4.0f))
+        return fail(MB_ERR_INVALID_ARGUMENT, "scan_margin must be in (0, 4]");
+    if (t->stage_pool != mb::kStagePool && t->stage_pool != mb::kStagePoolBig)
+        return fail(MB_ERR_INVALID_ARGUMENT, "stage_pool must be 896 or 1536");
+    p->scan_margin = t->scan_margin;
+    p->consts.scan_margin = t->scan_margin;
+    p->tuning.big_pool = t->stage_pool == mb::kStagePoolBig;
+    return MB_OK;
+}
+
+int mb_physics_autotune(mb_physics *p, uint32_t frames, mb_tuning *best, double *best_ms)
+{
+    if (!p) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    if (frames == 0) return fail(MB_ERR_INVALID_ARGUMENT, "frames_per_trial must be >= 1");
+    DeviceGuard g(p->device);
+    std::vector<char> ck(mb_checkpoint_size(p->n));
+    int rc = mb_physics_save_checkpoint(p, ck.data(), ck.size());
+    if (rc != MB_OK) return rc;
+    const mb_tuning keep{p->scan_margin, p->tuning.big_pool ? mb::kStagePoolBig : mb::kStagePool};
+    const float margins[] = {1.0f, 0.5f, 0.25f};
+    const uint32_t pools[] = {mb::kStagePool, mb::kStagePoolBig};
+    mb_tuning winner = keep;
+    double winner_ms = 0.0;
+    bool have = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    MB_HIP(hipEventCreate(&e0));
+    MB_HIP(hipEventCreate(&e1));
+    for (uint32_t pool : pools) {
+        for (float margin : margins) {
+            const mb_tuning t{margin, pool};
+            rc = mb_physics_load_checkpoint(p, ck.data(), ck.size());
+            if (rc == MB_OK) rc = mb_physics_set_tuning(p, &t);
+            if (rc == MB_OK && p->time < 0) rc = mb_physics_step(p, 1, nullptr);  // INIT
+            hipError_t e = hipSuccess;
+            if (rc == MB_OK) e = hipEventRecord(e0, nullptr);
+            if (rc == MB_OK && e == hipSuccess) rc = mb_physics_step(p, frames, nullptr);
+            if (rc == MB_OK && e == hipSuccess) e = hipEventRecord(e1, nullptr);
+            if (rc == MB_OK && e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (rc == MB_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (rc == MB_OK && e != hipSuccess) rc = fail_hip(e, "autotune timing");
+            if (rc != MB_OK) {
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                (void)mb_physics_load_checkpoint(p, ck.data(), ck.size());
+                (void)mb_physics_set_tuning(p, &keep);
+                return rc;
+            }
+            const double per = (double)ms / frames;
+            if (!have || per < winner_ms) {
+                winner = t;
+                winner_ms = per;
+                have = true;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    rc = mb_physics_load_checkpoint(p, ck.data(), ck.size());
+    if (rc != MB_OK) return rc;
+    rc = mb_physics_set_tuning(p, &winner);
+    if (rc != MB_OK) return rc;
+    if (best) *best = winner;
+    if (best_ms) *best_ms = winner_ms;
     return MB_OK;
 }
 

@@ -65,6 +65,8 @@ struct PhysConsts {
     int32_t enable_spring;
     FastDiv grid_size_div;   // u32 / GRID_SIZE
     FastDiv grids_div;       // cell key / NUMBER_OF_GRIDS_ONE_DIMENSION = row
+    // Scheduling choices that do not change results (autotuned):
+    float scan_margin;       // candidate-scan margin, as a fraction of the radius (> 0)
 };
 
 // ---- radix sort (8-bit digits, reduce-then-scan per pass) ----------------
@@ -103,6 +105,7 @@ hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
 constexpr uint32_t kMaxRows = 4096;     // G <= 4096 (G^2 <= 2^24)
 constexpr uint32_t kRowCap = 4096;      // spheres of one row sorted in LDS
 constexpr uint32_t kStagePool = 896;    // staged slots of a block's three neighbour-row windows
+constexpr uint32_t kStagePoolBig = 1536; // the same for dense scenes (fewer blocks per CU)
 
 // Row buckets.  The collide (or INIT) kernel of frame t files every sphere's
 // (column, slot) pair under the row of its next cell; the row sort of frame
@@ -149,12 +152,17 @@ struct FrameBuffers {
     uint4 *vfme_o;
 };
 
+// Launch choices of a frame that do not change results.
+struct FrameTuning {
+    bool big_pool = false;   // collide stages kStagePoolBig slots
+};
+
 // Enqueues frame `time`: INIT at time 0 (offsets, copy, keys, buckets for
 // frame 1), else LOOP (row sort of bucket set time & 1, collide + integrate
 // filing set !(time & 1)).  `events` (3 entries or null) are recorded before
 // the row sort, before collide + integrate and after it.
 hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time, uint32_t n,
-                        hipStream_t stream, hipEvent_t *events);
+                        hipStream_t stream, hipEvent_t *events, FrameTuning tuning = {});
 
 // Zeroes bucket set `set`, then writes key_out[i] = cell key of pos[i] and
 // files every sphere (slot i) in that set.

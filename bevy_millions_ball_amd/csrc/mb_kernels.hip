@@ -1213,7 +1213,7 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
     auto lds_of = [&](uint32_t j) {
         return j + r.lds0 + (j >= r.n0 ? r.lds1 : 0u) + (j >= r.n01 ? r.lds2 : 0u);
     };
-    const float margin = a.scale;
+    const float margin = c.scan_margin * a.scale;
     const float still = (margin * 0.999f) * (margin * 0.999f);
     const float own_reach = a.scale + margin;  // reach of candidate b: r_b + own_reach
     const uint32_t t_self = s - r.slot0;       // own slot, always in the middle range
@@ -1266,7 +1266,7 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
 // ranges (physics.wgsl:119-206) in the shader's order.  The block's first and
 // last cells come from the sorted keys, so the window bounds are uniform
 // loads issued at launch and one barrier separates staging from the walk.
-template <bool SPRING>
+template <bool SPRING, uint32_t POOL>
 __device__ __forceinline__ void physics_step_body(
     const PhysConsts &c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
     uint32_t *__restrict__ keys, const float4 *__restrict__ pos_i,
@@ -1325,7 +1325,7 @@ __device__ __forceinline__ void physics_step_body(
     }
     // the three windows share one pool, back to back
     const uint32_t wbase[3] = {0u, wcnt[0], wcnt[0] + wcnt[1]};
-    const bool staged = wcnt[0] + wcnt[1] + wcnt[2] <= kStagePool;
+    const bool staged = wcnt[0] + wcnt[1] + wcnt[2] <= POOL;
     if (staged) {
 #pragma unroll
         for (uint32_t w = 0; w < 3; ++w) {
@@ -1354,17 +1354,17 @@ __device__ __forceinline__ void physics_step_body(
                fast_div(kfirst, c.grids_div), nz, next - nz * G, s0, last);
 }
 
-template <bool SPRING>
+template <bool SPRING, uint32_t POOL>
 __global__ __launch_bounds__(256) void physics_step_kernel(
     PhysConsts c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
     uint32_t *__restrict__ keys, const float4 *__restrict__ pos_i,
     const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
     RowBuckets rb, uint32_t set_out, uint32_t n)
 {
-    __shared__ float4 s_pos[kStagePool];  // the three windows back to back
-    __shared__ uint2 s_vm[kStagePool];    // their velocity and mass halves
-    physics_step_body<SPRING>(c, perm, off, keys, pos_i, vfme_i, pos_o, vfme_o, rb, set_out, n,
-                              xcd_block(blockIdx.x, gridDim.x), s_pos, s_vm);
+    __shared__ float4 s_pos[POOL];  // the three windows back to back
+    __shared__ uint2 s_vm[POOL];    // their velocity and mass halves
+    physics_step_body<SPRING, POOL>(c, perm, off, keys, pos_i, vfme_i, pos_o, vfme_o, rb, set_out,
+                                    n, xcd_block(blockIdx.x, gridDim.x), s_pos, s_vm);
 }
 
 // INIT frame of the frame path: the sort of the all-zero keys is the
@@ -1432,8 +1432,23 @@ __global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *
     bucket_file(rb, rb.fill + set * c.grids * kLine, rb.spill_count + set * kLine, cz, cx, s);
 }
 
+template <uint32_t POOL>
+static void launch_collide(const FrameBuffers &b, const PhysConsts &c, uint32_t set_out,
+                           uint32_t n, hipStream_t stream)
+{
+    const dim3 grid((n + 255u) / 256u);
+    if (c.enable_spring)
+        hipLaunchKernelGGL((physics_step_kernel<true, POOL>), grid, dim3(256), 0, stream, c, b.perm,
+                           b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, b.rb, set_out,
+                           n);
+    else
+        hipLaunchKernelGGL((physics_step_kernel<false, POOL>), grid, dim3(256), 0, stream, c,
+                           b.perm, b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, b.rb,
+                           set_out, n);
+}
+
 hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time, uint32_t n,
-                        hipStream_t stream, hipEvent_t *events)
+                        hipStream_t stream, hipEvent_t *events, FrameTuning tuning)
 {
     if (n == 0) return hipSuccess;
     const uint32_t G = c.grids;
@@ -1458,14 +1473,10 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time
     hipLaunchKernelGGL(row_sort_kernel, dim3(G), dim3(256), sort_lds, stream, b.rb, set,
                        b.scratch_pairs, b.scratch_idx, G, n, b.perm, b.off, b.keys_out);
     if ((e = mark(1)) != hipSuccess) return e;
-    if (c.enable_spring) {
-        hipLaunchKernelGGL(physics_step_kernel<true>, grid, dim3(256), 0, stream, c, b.perm, b.off,
-                           b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, b.rb, set ^ 1u, n);
-    } else {
-        hipLaunchKernelGGL(physics_step_kernel<false>, grid, dim3(256), 0, stream, c, b.perm,
-                           b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, b.rb, set ^ 1u,
-                           n);
-    }
+    if (tuning.big_pool)
+        launch_collide<kStagePoolBig>(b, c, set ^ 1u, n, stream);
+    else
+        launch_collide<kStagePool>(b, c, set ^ 1u, n, stream);
     if ((e = mark(2)) != hipSuccess) return e;
     return hipGetLastError();
 }
@@ -1539,7 +1550,7 @@ __global__ __launch_bounds__(256) void batch_physics_step_kernel(
         return;
     }
     const uint32_t p = (uint32_t)S.time & 1u;
-    physics_step_body<SPRING>(S.c, gptr(S.perm), gptr(S.off), gptr(S.keys), gptr(S.pos[p]),
+    physics_step_body<SPRING, kStagePool>(S.c, gptr(S.perm), gptr(S.off), gptr(S.keys), gptr(S.pos[p]),
                               gptr(S.vfme[p]), gptr(S.pos[p ^ 1u]), gptr(S.vfme[p ^ 1u]), rb,
                               p ^ 1u, S.n, m.y, s_pos, s_vm);
 }

@@ -15,7 +15,7 @@ from typing import Optional, Tuple
 
 import numpy as np
 
-from ._lib import MB_INVALID_INDEX, MbBuffers, MbError, MbSettings, check, lib
+from ._lib import MB_INVALID_INDEX, MbBuffers, MbError, MbSettings, MbTuning, check, lib
 
 NUMBER_OF_THREADS_PER_WORKGROUP = 256  # mod.rs:35
 
@@ -226,6 +226,25 @@ class Physics:
         vf = np.empty((self.number_of_agents, 4), np.uint32)
         check(lib().mb_physics_download_state(self._h, _ptr(pos), _ptr(vf), _stream_ptr(stream)))
         return pos, vf
+
+    # -- tuning (launch choices that do not change results) ---------------
+    @property
+    def tuning(self) -> Tuple[float, int]:
+        t = MbTuning()
+        check(lib().mb_physics_get_tuning(self._h, C.byref(t)))
+        return float(t.scan_margin), int(t.stage_pool)
+
+    def set_tuning(self, scan_margin: float, stage_pool: int) -> None:
+        t = MbTuning(float(scan_margin), int(stage_pool))
+        check(lib().mb_physics_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, frames_per_trial: int = 5) -> Tuple[Tuple[float, int], float]:
+        """Times every candidate tuning on a copy of the state, keeps the
+        fastest; returns ((scan_margin, stage_pool), ms per frame)."""
+        t = MbTuning()
+        ms = C.c_double()
+        check(lib().mb_physics_autotune(self._h, int(frames_per_trial), C.byref(t), C.byref(ms)))
+        return (float(t.scan_margin), int(t.stage_pool)), ms.value
 
     # -- checkpoints --------------------------------------------------------
     def save_checkpoint(self) -> bytes:

@@ -104,6 +104,28 @@ int mb_physics_upload_state_device(mb_physics *physics,
 int mb_physics_download_state(const mb_physics *physics, float *position_with_scale,
                               uint32_t *packed_vfme, void *stream);
 
+/* Tuning (no counterpart in the reference): launch choices that do not
+ * change any result, only the speed.  scan_margin is the candidate-scan
+ * margin as a fraction of a sphere's radius (wider: more candidates reach
+ * the exact test; narrower: the scan restarts more often), in (0, 4];
+ * stage_pool is the number of neighbour slots a collide block stages in LDS,
+ * 896 (7 blocks per CU) or 1536 (4 blocks per CU; fewer blocks of dense
+ * scenes fall back to reading neighbours from global memory).  Defaults:
+ * 1.0 and 896. */
+typedef struct mb_tuning {
+    float scan_margin;
+    uint32_t stage_pool;
+} mb_tuning;
+
+int mb_physics_get_tuning(const mb_physics *physics, mb_tuning *tuning);
+int mb_physics_set_tuning(mb_physics *physics, const mb_tuning *tuning);
+/* Times `frames_per_trial` LOOP frames (after an INIT frame if none has run)
+ * for each candidate tuning on a copy of the current state, restores that
+ * state and time, applies the fastest tuning and returns it with its mean
+ * device time per frame (ms).  Synchronous. */
+int mb_physics_autotune(mb_physics *physics, uint32_t frames_per_trial, mb_tuning *best,
+                        double *best_ms);
+
 /* Checkpoints (no counterpart in the reference, whose state lives only in
  * its GPU buffers).  A checkpoint is a little-endian byte buffer: the magic
  * "MBCK", format version 1 (u32), mb_settings (60 bytes), time (i64), N (u32),

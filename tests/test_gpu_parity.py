@@ -336,6 +336,63 @@ def test_checkpoint_resume_is_bit_exact(gpu, split):
         assert np.array_equal(res[1], want[1])
 
 
+TUNINGS = [(m, p) for p in (896, 1536) for m in (1.0, 0.5, 0.25)]
+
+
+@pytest.mark.parametrize("scene", ["uniform", "dense"])
+def test_tuning_does_not_change_results(gpu, scene):
+    """Every tuning choice gives the default's results, bit for bit."""
+    from scene_helpers import dense_row_scene
+
+    if scene == "uniform":
+        cfg = replace(CONFIGS["uniform_1m"], number_of_agents=20_000, grids=128)
+        ps = cfg.settings()
+        pos, vf = generate(cfg, seed=9)
+    else:
+        ps, pos, vf = dense_row_scene()
+    runs = []
+    for margin, pool in TUNINGS:
+        with Physics(ps) as g:
+            g.set_tuning(margin, pool)
+            assert g.tuning == (margin, pool)
+            g.upload_state(pos, vf)
+            g.run(6)
+            p2, v2 = g.download_state()
+            runs.append((p2.view(np.uint32), v2, g.debug()["next_keys"], (margin, pool)))
+    for p2, v2, k2, t in runs[1:]:
+        assert np.array_equal(p2, runs[0][0]), t
+        assert np.array_equal(v2, runs[0][1]), t
+        assert np.array_equal(k2, runs[0][2]), t
+
+
+def test_autotune_restores_state_and_keeps_results(gpu):
+    cfg = replace(CONFIGS["uniform_1m"], number_of_agents=20_000, grids=128)
+    ps = cfg.settings()
+    pos, vf = generate(cfg, seed=3)
+    with Physics(ps) as a, Physics(ps) as b:
+        for g in (a, b):
+            g.upload_state(pos, vf)
+            g.run(3)
+        before = a.download_state()
+        best, ms = a.autotune(2)
+        assert best in TUNINGS and ms > 0
+        assert a.tuning == best and a.time() == b.time() == 2
+        after = a.download_state()
+        assert np.array_equal(before[0].view(np.uint32), after[0].view(np.uint32))
+        assert np.array_equal(before[1], after[1])
+        a.run(4)
+        b.run(4)
+        pa, va = a.download_state()
+        pb, vb = b.download_state()
+        assert np.array_equal(pa.view(np.uint32), pb.view(np.uint32)) and np.array_equal(va, vb)
+    with pytest.raises(Exception, match="scan_margin"):
+        with Physics(ps) as c:
+            c.set_tuning(0.0, 896)
+    with pytest.raises(Exception, match="stage_pool"):
+        with Physics(ps) as c:
+            c.set_tuning(1.0, 1000)
+
+
 def test_checkpoint_rejects_other_sizes(gpu):
     cfg = replace(CONFIGS["plain_65k"], number_of_agents=4096, grids=64)
     with Physics(cfg.settings()) as a:
