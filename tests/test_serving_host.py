@@ -21,6 +21,7 @@ STRUCTS = {
     "mb_server_config": _lib.MbServerConfig,
     "mb_request_status": _lib.MbRequestStatus,
     "mb_server_stats": _lib.MbServerStats,
+    "mb_tuning": _lib.MbTuning,
 }
 
 
