@@ -177,8 +177,11 @@ hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t count, hipStrea
 struct SceneDesc {
     PhysConsts c;
     uint32_t n;
-    int32_t time;          // time of the frame about to run; 0 runs INIT
-    uint32_t frames_left;  // frames still to run; 0 = idle (its blocks return at once)
+    // At frame k of the current plan (k = 0 for the first frame after the
+    // plan was written) the scene runs its frame time0 + k, INIT when that is
+    // 0, while k < frames0; afterwards its blocks return at once.
+    int32_t time0;
+    uint32_t frames0;
     uint32_t pad_;
     float4 *pos[2];        // state ping-pong: frame t reads [t & 1], writes [!(t & 1)]
     uint4 *vfme[2];
@@ -195,6 +198,7 @@ struct SceneDesc {
 struct BatchPlan {
     SceneDesc *scenes;
     uint32_t scene_count;
+    uint32_t *frame_counter;  // graph replays: k in device memory, advanced by the frame
     const uint2 *row_map;     // rows of the grid
     uint32_t rows;
     const uint2 *blk_plain;   // blocks of 256 sorted slots, spring off
@@ -204,9 +208,12 @@ struct BatchPlan {
     uint32_t max_grids;       // largest G among the scenes (row sort LDS)
 };
 
-// Enqueues one frame of every scene with frames_left > 0 (INIT for those at
-// time 0), then advances their time and frames_left on the device.
-hipError_t launch_batch_frame(const BatchPlan &p, hipStream_t stream);
+// Enqueues frame k of the plan for every scene (INIT for those at time 0).
+// With from_counter, k is read from p.frame_counter and the frame ends with
+// a one-thread kernel that advances it (so that a captured frame can be
+// replayed unchanged); otherwise k is a kernel argument.
+hipError_t launch_batch_frame(const BatchPlan &p, uint32_t k, bool from_counter,
+                              hipStream_t stream);
 
 // dst[i] = src[i] for i < words, by a kernel (src may be mapped host memory).
 hipError_t launch_copy_words(uint32_t *dst, const uint32_t *src, uint32_t words,

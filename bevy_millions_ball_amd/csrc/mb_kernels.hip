@@ -1510,16 +1510,30 @@ __device__ __forceinline__ RowBuckets scene_buckets(const SceneDesc &S)
     return rb;
 }
 
+// The scene's time at plan frame k, or -1 when it has no frame to run.
+__device__ __forceinline__ int32_t scene_time(const SceneDesc &S, uint32_t k)
+{
+    return k < S.frames0 ? S.time0 + (int32_t)k : -1;
+}
+
+__device__ __forceinline__ uint32_t plan_frame(uint32_t k, const uint32_t *counter)
+{
+    return counter ? *counter : k;
+}
+
 __global__ __launch_bounds__(256) void batch_row_sort_kernel(const SceneDesc *__restrict__ scenes,
-                                                             const uint2 *__restrict__ map)
+                                                             const uint2 *__restrict__ map,
+                                                             uint32_t k,
+                                                             const uint32_t *__restrict__ counter)
 {
     const uint2 m = map[xcd_block(blockIdx.x, gridDim.x)];
     const SceneDesc &S = scenes[m.x];
-    if (S.frames_left == 0u) return;
+    const int32_t time = scene_time(S, plan_frame(k, counter));
+    if (time < 0) return;
     const uint32_t G = S.c.grids;
     uint32_t *off = gptr(S.off);
     const RowBuckets rb = scene_buckets(S);
-    if (S.time == 0) {
+    if (time == 0) {
         // INIT: cell 0 holds every sphere; clear bucket set 1 for the INIT step
         for (uint32_t c = threadIdx.x; c < G; c += 256u)
             off[m.y * G + c] = (m.y == 0u && c == 0u) ? 0u : S.n;
@@ -1530,40 +1544,36 @@ __global__ __launch_bounds__(256) void batch_row_sort_kernel(const SceneDesc *__
         }
         return;
     }
-    row_sort_body(rb, (uint32_t)S.time & 1u, gptr(S.scratch_pairs), gptr(S.scratch_idx), G, S.n,
+    row_sort_body(rb, (uint32_t)time & 1u, gptr(S.scratch_pairs), gptr(S.scratch_idx), G, S.n,
                   gptr(S.perm), off, gptr(S.keys), m.y);
 }
 
 template <bool SPRING>
 __global__ __launch_bounds__(256) void batch_physics_step_kernel(
-    const SceneDesc *__restrict__ scenes, const uint2 *__restrict__ map)
+    const SceneDesc *__restrict__ scenes, const uint2 *__restrict__ map, uint32_t k,
+    const uint32_t *__restrict__ counter)
 {
     __shared__ float4 s_pos[kStagePool];
     __shared__ uint2 s_vm[kStagePool];
     const uint2 m = map[xcd_block(blockIdx.x, gridDim.x)];
     const SceneDesc &S = scenes[m.x];
-    if (S.frames_left == 0u) return;
+    const int32_t time = scene_time(S, plan_frame(k, counter));
+    if (time < 0) return;
     const RowBuckets rb = scene_buckets(S);
-    if (S.time == 0) {
+    if (time == 0) {
         physics_init_body(S.c, gptr(S.perm), gptr(S.keys), gptr(S.pos[0]), gptr(S.vfme[0]),
                           gptr(S.pos[1]), gptr(S.vfme[1]), rb, S.n, m.y * 256u + threadIdx.x);
         return;
     }
-    const uint32_t p = (uint32_t)S.time & 1u;
+    const uint32_t p = (uint32_t)time & 1u;
     physics_step_body<SPRING, kStagePool>(S.c, gptr(S.perm), gptr(S.off), gptr(S.keys), gptr(S.pos[p]),
                               gptr(S.vfme[p]), gptr(S.pos[p ^ 1u]), gptr(S.vfme[p ^ 1u]), rb,
                               p ^ 1u, S.n, m.y, s_pos, s_vm);
 }
 
-__global__ __launch_bounds__(256) void batch_advance_kernel(SceneDesc *__restrict__ scenes,
-                                                            uint32_t count)
+__global__ __launch_bounds__(64) void batch_advance_kernel(uint32_t *__restrict__ counter)
 {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= count) return;
-    const uint32_t left = scenes[i].frames_left;
-    if (left == 0u) return;
-    scenes[i].time += 1;
-    scenes[i].frames_left = left - 1u;
+    if (threadIdx.x == 0) *counter += 1u;
 }
 
 // Plan tables from mapped, coherent host memory into device memory: a kernel
@@ -1585,22 +1595,24 @@ hipError_t launch_copy_words(uint32_t *dst, const uint32_t *src, uint32_t words,
     return hipGetLastError();
 }
 
-hipError_t launch_batch_frame(const BatchPlan &p, hipStream_t stream)
+hipError_t launch_batch_frame(const BatchPlan &p, uint32_t k, bool from_counter,
+                              hipStream_t stream)
 {
+    const uint32_t *counter = from_counter ? p.frame_counter : nullptr;
     if (p.scene_count == 0) return hipSuccess;
     if (p.rows) {
         const size_t sort_lds = ((size_t)((p.max_grids + 1023u) & ~1023u) + kRowCap) * 4u;
         hipLaunchKernelGGL(batch_row_sort_kernel, dim3(p.rows), dim3(256), sort_lds, stream,
-                           p.scenes, p.row_map);
+                           p.scenes, p.row_map, k, counter);
     }
     if (p.blk_plain_count)
         hipLaunchKernelGGL(batch_physics_step_kernel<false>, dim3(p.blk_plain_count), dim3(256), 0,
-                           stream, p.scenes, p.blk_plain);
+                           stream, p.scenes, p.blk_plain, k, counter);
     if (p.blk_spring_count)
         hipLaunchKernelGGL(batch_physics_step_kernel<true>, dim3(p.blk_spring_count), dim3(256), 0,
-                           stream, p.scenes, p.blk_spring);
-    hipLaunchKernelGGL(batch_advance_kernel, dim3((p.scene_count + 255u) / 256u), dim3(256), 0,
-                       stream, p.scenes, p.scene_count);
+                           stream, p.scenes, p.blk_spring, k, counter);
+    if (from_counter)
+        hipLaunchKernelGGL(batch_advance_kernel, dim3(1), dim3(64), 0, stream, p.frame_counter);
     return hipGetLastError();
 }
 

@@ -5,12 +5,13 @@
 //   - Device memory comes from a chunked arena (first fit, coalescing free
 //     lists); each scene holds one region with all of its arrays.  Admission
 //     is first come, first served against a byte cap on the regions.
-//   - Each tick runs one batched frame (mb::launch_batch_frame): five
-//     launches cover every resident scene, whatever their sizes and grids.
-//     Per-scene time and frames_left live in the device scene table and are
-//     advanced by the frame's last kernel, so an unchanged composition can
-//     be replayed as one captured hipGraph.  The host keeps a mirror of both
-//     counters and rewrites the table only when the composition changes.
+//   - Each tick runs one batched frame (mb::launch_batch_frame): two or
+//     three launches cover every resident scene, whatever their sizes and
+//     grids.  The scene table holds each scene's time and frame count at the
+//     plan's first frame; a frame index k (a kernel argument, or a device
+//     counter advanced by the frame itself when a captured hipGraph is
+//     replayed) gives the current ones.  The host rewrites the table only
+//     when the composition changes.
 //   - Uploads and retirement downloads run on two more streams, so they
 //     overlap other scenes' frames.  An admitted scene joins the frames once
 //     its upload has completed; a retired scene's download waits for its
@@ -228,8 +229,8 @@ struct Request {
         mb::SceneDesc d{};
         d.c = consts;
         d.n = n;
-        d.time = time_next;
-        d.frames_left = frames_left;
+        d.time0 = time_next;
+        d.frames0 = frames_left;
         for (int i = 0; i < 2; ++i) {
             d.pos[i] = reinterpret_cast<float4 *>(region + layout.pos[i]);
             d.vfme[i] = reinterpret_cast<uint4 *>(region + layout.vfme[i]);
@@ -298,6 +299,7 @@ struct mb_server {
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     int stage_k = 0;
     mb::BatchPlan plan{};
+    uint32_t plan_k = 0;  // frames run since the plan was written
 
     hipGraphExec_t exec = nullptr;
     std::vector<std::pair<hipGraphExec_t, hipEvent_t>> graveyard;
@@ -472,9 +474,11 @@ int upload_plan(mb_server *s)
         auto &blk = r->consts.enable_spring ? spring : plain;
         for (uint32_t b = 0; b < (r->n + 255u) / 256u; ++b) blk.push_back(make_uint2(i, b));
     }
+    // staged plan: frame counter (one aligned slot) | scene table | work lists
+    const size_t head = kAlign;
     const size_t desc_bytes = align_up(descs.size() * sizeof(mb::SceneDesc));
     const size_t map_count = rows.size() + plain.size() + spring.size();
-    const size_t need = desc_bytes + map_count * sizeof(uint2);
+    const size_t need = head + desc_bytes + map_count * sizeof(uint2);
     if (s->plan_cap < need) {
         // grow geometrically; frames in flight may still read the old buffer
         if (s->d_plan) s->old_device.push_back(s->d_plan);
@@ -495,8 +499,9 @@ int upload_plan(mb_server *s)
                              hipHostMallocMapped | hipHostMallocCoherent));
     }
     char *st = static_cast<char *>(s->stage[k]);
-    std::memcpy(st, descs.data(), descs.size() * sizeof(mb::SceneDesc));
-    uint2 *m = reinterpret_cast<uint2 *>(st + desc_bytes);
+    std::memset(st, 0, head);  // the frame counter starts at 0
+    std::memcpy(st + head, descs.data(), descs.size() * sizeof(mb::SceneDesc));
+    uint2 *m = reinterpret_cast<uint2 *>(st + head + desc_bytes);
     size_t at = 0;
     for (auto *v : {&rows, &plain, &spring}) {
         std::memcpy(m + at, v->data(), v->size() * sizeof(uint2));
@@ -511,9 +516,11 @@ int upload_plan(mb_server *s)
     MB_HIP(hipEventRecord(s->stage_ev[k], s->compute));
 
     mb::BatchPlan &p = s->plan;
-    p.scenes = reinterpret_cast<mb::SceneDesc *>(s->d_plan);
+    p.frame_counter = reinterpret_cast<uint32_t *>(s->d_plan);
+    p.scenes = reinterpret_cast<mb::SceneDesc *>(s->d_plan + head);
     p.scene_count = count;
-    p.row_map = reinterpret_cast<const uint2 *>(s->d_plan + desc_bytes);
+    p.row_map = reinterpret_cast<const uint2 *>(s->d_plan + head + desc_bytes);
+    s->plan_k = 0;
     p.rows = (uint32_t)rows.size();
     p.blk_plain = p.row_map + rows.size();
     p.blk_plain_count = (uint32_t)plain.size();
@@ -531,7 +538,7 @@ int upload_plan(mb_server *s)
         }
         hipGraph_t graph = nullptr;
         MB_HIP(hipStreamBeginCapture(s->compute, hipStreamCaptureModeThreadLocal));
-        const hipError_t le = mb::launch_batch_frame(p, s->compute);
+        const hipError_t le = mb::launch_batch_frame(p, 0, true, s->compute);
         const hipError_t ce = hipStreamEndCapture(s->compute, &graph);
         if (le != hipSuccess) return fail_hip(le, "batched frame capture");
         if (ce != hipSuccess) return fail_hip(ce, "hipStreamEndCapture");
@@ -762,7 +769,7 @@ int mb_server_tick(mb_server *s, uint32_t *running)
     if (s->config.use_graph) {
         MB_HIP(hipGraphLaunch(s->exec, s->compute));
     } else {
-        hipError_t e = mb::launch_batch_frame(s->plan, s->compute);
+        hipError_t e = mb::launch_batch_frame(s->plan, s->plan_k, false, s->compute);
         if (e != hipSuccess) return fail_hip(e, "batched frame");
     }
     if (t1) {
@@ -774,6 +781,7 @@ int mb_server_tick(mb_server *s, uint32_t *running)
     MB_HIP(hipEventRecord(fe, s->compute));
     s->inflight.push_back(fe);
     ++s->frames;
+    ++s->plan_k;
     if (running) *running = (uint32_t)s->running.size();
 
     std::vector<Request *> still;
