@@ -42,9 +42,11 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x, uint32_t lan
 
 // Exclusive prefix sum over a 256-thread block; `total` receives the sum.
 // `scratch` holds 4 words of LDS.  Contains two barriers.
-__device__ __forceinline__ uint32_t block256_exclusive_scan(uint32_t v, uint32_t *scratch,
-                                                            uint32_t &total)
+template <uint32_t NT>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *scratch,
+                                                         uint32_t &total)
 {
+    constexpr uint32_t kWaves = NT / 64u;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
     uint32_t inc = wave_inclusive_scan(v, lane);
@@ -52,7 +54,7 @@ __device__ __forceinline__ uint32_t block256_exclusive_scan(uint32_t v, uint32_t
     __syncthreads();
     uint32_t before = 0, sum = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < 4; ++w) {
+    for (uint32_t w = 0; w < kWaves; ++w) {
         uint32_t t = scratch[w];
         before += (w < wave) ? t : 0u;
         sum += t;
@@ -60,6 +62,12 @@ __device__ __forceinline__ uint32_t block256_exclusive_scan(uint32_t v, uint32_t
     total = sum;
     __syncthreads();
     return before + inc - v;
+}
+
+__device__ __forceinline__ uint32_t block256_exclusive_scan(uint32_t v, uint32_t *scratch,
+                                                            uint32_t &total)
+{
+    return block_exclusive_scan<256>(v, scratch, total);
 }
 
 // Lanes of the wave whose 8-bit digit equals this lane's (ballot per bit).
@@ -836,23 +844,26 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb)
     return x * q + min(x, rem) + j;
 }
 
-// Exclusive scan, in place, of `count` (<= 4096) LDS words with a 256-thread
+// Exclusive scan, in place, of `count` (<= 4096) LDS words with an NT-thread
 // block; `vals` must be 16-byte aligned and hold `count` rounded up to a
-// multiple of 1024 words (the padding is scanned too, as zeros on entry is
-// not required: callers ignore it).  Each thread owns 4, 8 or 16 consecutive
-// words and moves them with 16-byte LDS accesses, which keeps every wave free
-// of bank conflicts.  Returns the total of the first `count` words.
-__device__ __forceinline__ uint32_t block256_scan_lds(uint32_t *vals, uint32_t count,
-                                                      uint32_t *scratch)
+// multiple of 4 NT words (the padding is scanned too; callers ignore it).
+// Each thread owns 4, 8 or 16 consecutive words and moves them with 16-byte
+// LDS accesses, which keeps every wave free of bank conflicts.  Returns the
+// total of the first `count` words.
+template <uint32_t NT>
+__device__ __forceinline__ uint32_t block_scan_lds(uint32_t *vals, uint32_t count,
+                                                   uint32_t *scratch)
 {
+    constexpr uint32_t kSpan = 4u * NT;               // words per pass of the block
+    constexpr uint32_t kMaxVecs = 4096u / kSpan;      // uint4 per thread at count 4096
     const uint32_t tid = threadIdx.x;
-    const uint32_t vecs = (count + 1023u) / 1024u;  // uint4 per thread: 1, 2 or 4
+    const uint32_t vecs = (count + kSpan - 1u) / kSpan;
     uint4 *v4 = reinterpret_cast<uint4 *>(vals);
     const uint32_t beg = tid * vecs * 4u;
-    uint4 q[4];
+    uint4 q[kMaxVecs];
     uint32_t sum = 0;
 #pragma unroll
-    for (uint32_t j = 0; j < 4; ++j) {
+    for (uint32_t j = 0; j < kMaxVecs; ++j) {
         if (j < vecs) {
             q[j] = v4[tid * vecs + j];
             // entries at or past `count` do not contribute
@@ -865,9 +876,9 @@ __device__ __forceinline__ uint32_t block256_scan_lds(uint32_t *vals, uint32_t c
         }
     }
     uint32_t total;
-    uint32_t run = block256_exclusive_scan(sum, scratch, total);
+    uint32_t run = block_exclusive_scan<NT>(sum, scratch, total);
 #pragma unroll
-    for (uint32_t j = 0; j < 4; ++j) {
+    for (uint32_t j = 0; j < kMaxVecs; ++j) {
         if (j < vecs) {
             uint4 o;
             o.x = run;
@@ -984,6 +995,13 @@ __device__ __forceinline__ void block_file(BlockFiling &f, const RowBuckets &rb,
     }
 }
 
+// LDS words of the cell counters of a row sort: `grids` rounded up to a
+// multiple of the block scan's span (4 NT); the kRowCap indices follow.
+__host__ __device__ constexpr uint32_t row_sort_cells(uint32_t grids, uint32_t nt)
+{
+    return (grids + 4u * nt - 1u) / (4u * nt) * (4u * nt);
+}
+
 // One block per row r of frame `time` (>= 1).  The row's pairs are in bucket
 // set time & 1 (spilled pairs in the spill list); its slots start after the
 // spheres of all lower rows.  Counting sort by column in LDS, then each index
@@ -993,6 +1011,7 @@ __device__ __forceinline__ void block_file(BlockFiling &f, const RowBuckets &rb,
 // collide kernel that follows.  Dynamic LDS: `grids` cell counters (padded
 // to a multiple of 1024 words for the vector scan) followed by kRowCap
 // indices.
+template <uint32_t NT>
 __device__ __forceinline__ void row_sort_body(
     const RowBuckets &rb, uint32_t set, uint64_t *__restrict__ scratch_pairs,
     uint32_t *__restrict__ scratch_idx, uint32_t grids, uint32_t n,
@@ -1001,30 +1020,30 @@ __device__ __forceinline__ void row_sort_body(
 {
     extern __shared__ __align__(16) uint32_t s_dyn[];
     uint32_t *s_cell = s_dyn;                              // grids words (padded)
-    uint32_t *s_idx = s_dyn + ((grids + 1023u) & ~1023u);  // kRowCap words
-    __shared__ uint32_t s_scratch[4];
+    uint32_t *s_idx = s_dyn + row_sort_cells(grids, NT);  // kRowCap words
+    __shared__ uint32_t s_scratch[NT / 64u];
     __shared__ uint32_t s_spilled;
     const uint32_t tid = threadIdx.x;
     const uint32_t *fill_in = rb.fill + set * grids * kLine;
     uint32_t part = 0;
-    for (uint32_t q = tid; q < r; q += 256u) part += fill_in[q * kLine];
+    for (uint32_t q = tid; q < r; q += NT) part += fill_in[q * kLine];
     uint32_t base;
-    (void)block256_exclusive_scan(part, s_scratch, base);  // base = spheres in rows < r
+    (void)block_exclusive_scan<NT>(part, s_scratch, base);  // base = spheres in rows < r
     const uint32_t m = fill_in[r * kLine];
     if (tid == 0) {
         rb.fill[((set ^ 1u) * grids + r) * kLine] = 0u;
         if (r == 0) rb.spill_count[(set ^ 1u) * kLine] = 0u;
         s_spilled = 0u;
     }
-    for (uint32_t c = tid; c < grids; c += 256u) s_cell[c] = 0u;
+    for (uint32_t c = tid; c < grids; c += NT) s_cell[c] = 0u;
     const uint64_t *src = rb.bucket + (size_t)r * rb.cap;
     if (m > rb.cap) {
         // the bucket and the row's spilled pairs, gathered in the row's slots
         // of the scratch array
-        for (uint32_t e = tid; e < rb.cap; e += 256u) scratch_pairs[base + e] = src[e];
+        for (uint32_t e = tid; e < rb.cap; e += NT) scratch_pairs[base + e] = src[e];
         __syncthreads();
         const uint32_t spilled = rb.spill_count[set * kLine];
-        for (uint32_t i = tid; i < spilled; i += 256u) {
+        for (uint32_t i = tid; i < spilled; i += NT) {
             const uint64_t v = rb.spill[i];
             if ((uint32_t)(v >> 42) == r) {
                 const uint32_t at = rb.cap + atomicAdd(&s_spilled, 1u);
@@ -1037,35 +1056,33 @@ __device__ __forceinline__ void row_sort_body(
     __syncthreads();
 
     if (m <= kRowCap) {
-        constexpr uint32_t kPer = kRowCap / 256u;
+        constexpr uint32_t kPer = kRowCap / NT;
         uint64_t el[kPer];
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) {
-            const uint32_t e = j * 256u + tid;
+            const uint32_t e = j * NT + tid;
             el[j] = e < m ? src[e] : 0ull;
         }
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) {
-            const uint32_t e = j * 256u + tid;
+            const uint32_t e = j * NT + tid;
             if (e < m) atomicAdd(&s_cell[(uint32_t)(el[j] >> 32)], 1u);
         }
         __syncthreads();
-        (void)block256_scan_lds(s_cell, grids, s_scratch);
-        for (uint32_t c = tid; c < grids; c += 256u) off[r * grids + c] = base + s_cell[c];
+        (void)block_scan_lds<NT>(s_cell, grids, s_scratch);
+        for (uint32_t c = tid; c < grids; c += NT) off[r * grids + c] = base + s_cell[c];
         __syncthreads();
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) {
-            const uint32_t e = j * 256u + tid;
+            const uint32_t e = j * NT + tid;
             if (e < m) s_idx[atomicAdd(&s_cell[(uint32_t)(el[j] >> 32)], 1u)] = (uint32_t)el[j];
         }
         __syncthreads();
         // s_cell[c] is now the end of cell c inside the row; rank each index
-        // among its cell's indices.
-        uint32_t dst[kPer];
+        // among its cell's indices and write its slot's perm and key there.
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) {
-            const uint32_t e = j * 256u + tid;
-            dst[j] = kInvalid;
+            const uint32_t e = j * NT + tid;
             if (e < m) {
                 const uint32_t c = (uint32_t)(el[j] >> 32);
                 const uint32_t idx = (uint32_t)el[j];
@@ -1073,36 +1090,26 @@ __device__ __forceinline__ void row_sort_body(
                 const uint32_t ce = s_cell[c];
                 uint32_t rank = 0;
                 for (uint32_t q = cs; q < ce; ++q) rank += s_idx[q] < idx;
-                dst[j] = cs + rank;
+                perm[base + cs + rank] = idx;
+                sorted_keys[base + cs + rank] = r * grids + c;
             }
-        }
-        __syncthreads();
-#pragma unroll
-        for (uint32_t j = 0; j < kPer; ++j)
-            if (dst[j] != kInvalid) s_idx[dst[j]] = (uint32_t)el[j];
-        __syncthreads();
-        for (uint32_t e = tid; e < m; e += 256u) perm[base + e] = s_idx[e];
-        // sorted keys: the cell of slot base + e is the cell whose end exceeds e
-        for (uint32_t c = tid; c < grids; c += 256u) {
-            const uint32_t cs = c ? s_cell[c - 1u] : 0u;
-            for (uint32_t e = cs; e < s_cell[c]; ++e) sorted_keys[base + e] = r * grids + c;
         }
     } else {
         // A row larger than the LDS tile: the same steps with the scattered
         // indices in global scratch.
-        for (uint32_t e = tid; e < m; e += 256u)
+        for (uint32_t e = tid; e < m; e += NT)
             atomicAdd(&s_cell[(uint32_t)(src[e] >> 32)], 1u);
         __syncthreads();
-        (void)block256_scan_lds(s_cell, grids, s_scratch);
-        for (uint32_t c = tid; c < grids; c += 256u) off[r * grids + c] = base + s_cell[c];
+        (void)block_scan_lds<NT>(s_cell, grids, s_scratch);
+        for (uint32_t c = tid; c < grids; c += NT) off[r * grids + c] = base + s_cell[c];
         __syncthreads();
-        for (uint32_t e = tid; e < m; e += 256u) {
+        for (uint32_t e = tid; e < m; e += NT) {
             const uint64_t v = src[e];
             scratch_idx[base + atomicAdd(&s_cell[(uint32_t)(v >> 32)], 1u)] = (uint32_t)v;
         }
         __threadfence_block();
         __syncthreads();
-        for (uint32_t e = tid; e < m; e += 256u) {
+        for (uint32_t e = tid; e < m; e += NT) {
             const uint64_t v = src[e];
             const uint32_t c = (uint32_t)(v >> 32);
             const uint32_t idx = (uint32_t)v;
@@ -1117,13 +1124,18 @@ __device__ __forceinline__ void row_sort_body(
     if (tid == 0 && r == grids - 1u) off[grids * grids] = n;
 }
 
-__global__ __launch_bounds__(256) void row_sort_kernel(
+__global__ __launch_bounds__(kSortThreads) void row_sort_kernel(
     RowBuckets rb, uint32_t set, uint64_t *__restrict__ scratch_pairs,
     uint32_t *__restrict__ scratch_idx, uint32_t grids, uint32_t n,
     uint32_t *__restrict__ perm, uint32_t *__restrict__ off, uint32_t *__restrict__ sorted_keys)
 {
-    row_sort_body(rb, set, scratch_pairs, scratch_idx, grids, n, perm, off, sorted_keys,
-                  xcd_block(blockIdx.x, gridDim.x));
+    row_sort_body<kSortThreads>(rb, set, scratch_pairs, scratch_idx, grids, n, perm, off,
+                                sorted_keys, xcd_block(blockIdx.x, gridDim.x));
+}
+
+static size_t row_sort_lds(uint32_t grids)
+{
+    return ((size_t)row_sort_cells(grids, kSortThreads) + kRowCap) * 4u;
 }
 
 // solve_sphere_constraint behind a cheap conservative filter:
@@ -1469,8 +1481,7 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time
         return hipGetLastError();
     }
     const uint32_t set = (uint32_t)(time & 1);
-    const size_t sort_lds = ((size_t)((G + 1023u) & ~1023u) + kRowCap) * 4u;
-    hipLaunchKernelGGL(row_sort_kernel, dim3(G), dim3(256), sort_lds, stream, b.rb, set,
+    hipLaunchKernelGGL(row_sort_kernel, dim3(G), dim3(kSortThreads), row_sort_lds(G), stream, b.rb, set,
                        b.scratch_pairs, b.scratch_idx, G, n, b.perm, b.off, b.keys_out);
     if ((e = mark(1)) != hipSuccess) return e;
     if (tuning.big_pool)
@@ -1521,7 +1532,7 @@ __device__ __forceinline__ uint32_t plan_frame(uint32_t k, const uint32_t *count
     return counter ? *counter : k;
 }
 
-__global__ __launch_bounds__(256) void batch_row_sort_kernel(const SceneDesc *__restrict__ scenes,
+__global__ __launch_bounds__(kSortThreads) void batch_row_sort_kernel(const SceneDesc *__restrict__ scenes,
                                                              const uint2 *__restrict__ map,
                                                              uint32_t k,
                                                              const uint32_t *__restrict__ counter)
@@ -1535,7 +1546,7 @@ __global__ __launch_bounds__(256) void batch_row_sort_kernel(const SceneDesc *__
     const RowBuckets rb = scene_buckets(S);
     if (time == 0) {
         // INIT: cell 0 holds every sphere; clear bucket set 1 for the INIT step
-        for (uint32_t c = threadIdx.x; c < G; c += 256u)
+        for (uint32_t c = threadIdx.x; c < G; c += kSortThreads)
             off[m.y * G + c] = (m.y == 0u && c == 0u) ? 0u : S.n;
         if (threadIdx.x == 0) {
             if (m.y == G - 1u) off[G * G] = S.n;
@@ -1544,7 +1555,7 @@ __global__ __launch_bounds__(256) void batch_row_sort_kernel(const SceneDesc *__
         }
         return;
     }
-    row_sort_body(rb, (uint32_t)time & 1u, gptr(S.scratch_pairs), gptr(S.scratch_idx), G, S.n,
+    row_sort_body<kSortThreads>(rb, (uint32_t)time & 1u, gptr(S.scratch_pairs), gptr(S.scratch_idx), G, S.n,
                   gptr(S.perm), off, gptr(S.keys), m.y);
 }
 
@@ -1601,8 +1612,8 @@ hipError_t launch_batch_frame(const BatchPlan &p, uint32_t k, bool from_counter,
     const uint32_t *counter = from_counter ? p.frame_counter : nullptr;
     if (p.scene_count == 0) return hipSuccess;
     if (p.rows) {
-        const size_t sort_lds = ((size_t)((p.max_grids + 1023u) & ~1023u) + kRowCap) * 4u;
-        hipLaunchKernelGGL(batch_row_sort_kernel, dim3(p.rows), dim3(256), sort_lds, stream,
+        hipLaunchKernelGGL(batch_row_sort_kernel, dim3(p.rows), dim3(kSortThreads),
+                           row_sort_lds(p.max_grids), stream,
                            p.scenes, p.row_map, k, counter);
     }
     if (p.blk_plain_count)
