@@ -1228,6 +1228,9 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
     const float margin = c.scan_margin * a.scale;
     const float still = (margin * 0.999f) * (margin * 0.999f);
     const float own_reach = a.scale + margin;  // reach of candidate b: r_b + own_reach
+    // the scan widens that reach by 1.0001 (its square by 1.0002, against the
+    // exact test's 1.0001), which also covers the fused d2 of the scan
+    const float own_reach_w = own_reach * 1.0001f;
     const uint32_t t_self = s - r.slot0;       // own slot, always in the middle range
     // byte offset of candidate j's staged position
     const uint32_t lb0 = r.lds0 * 16u, lb1 = r.lds1 * 16u, lb2 = r.lds2 * 16u;
@@ -1236,24 +1239,27 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
     while (j < total) {
         const uint32_t end = min(j + 32u, total);
         const float rx = a.px, ry = a.py, rz = a.pz;
+        // candidate t of the chunk sets bit end - 1 - t: the first one is the
+        // highest bit, so a shift-and-add builds the mask
         uint32_t near = 0;
-        for (uint32_t t = j, k = 0; t < end; ++t, ++k) {
+        for (uint32_t t = j; t < end; ++t) {
             float4 bp;
             if (FROM_LDS)
                 bp = *reinterpret_cast<const float4 *>(
                     s_pos_b + (t * 16u + lb0 + (t >= r.n0 ? lb1 : 0u) + (t >= r.n01 ? lb2 : 0u)));
             else
                 bp = pos_i[perm[slot_of(t)]];
-            const float reach = bp.w + own_reach;
+            const float reach = __builtin_fmaf(bp.w, 1.0001f, own_reach_w);
             const float dx = rx - bp.x, dy = ry - bp.y, dz = rz - bp.z;
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            near |= (uint32_t)(d2 <= (reach * reach) * 1.0001f) << k;
+            const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+            near = near + near + (d2 <= reach * reach ? 1u : 0u);
         }
-        if (t_self >= j && t_self < end) near &= ~(1u << (t_self - j));
+        if (t_self >= j && t_self < end) near &= ~(1u << (end - 1u - t_self));
         uint32_t next = end;
         while (near) {
-            const uint32_t t = j + (uint32_t)__builtin_ctz(near);
-            near &= near - 1u;
+            const uint32_t lz = (uint32_t)__builtin_clz(near);
+            const uint32_t t = end - 32u + lz;
+            near ^= 0x80000000u >> lz;
             if (FROM_LDS) {
                 const uint32_t l = lds_of(t);
                 collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
