@@ -70,7 +70,7 @@ def cpu_baseline(cfg, pos, vf, budget_s: float, threads: int) -> dict:
         w.step(1, threads=threads)
         frames += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or frames >= 200:
+        if el >= budget_s or frames >= 2000:
             break
     w.close()
     return {
@@ -91,7 +91,9 @@ def serving_bench(n_requests: int, n_poisson: int, seed: int) -> dict:
     1-60 LOOP steps, plain-step distributions (bevy_millions_ball_amd.serving).
     The timed region of both throughput legs is the server's work after the
     requests were handed over: admission, host->device upload, INIT + LOOP
-    frames, device->host retirement.  Throughput counts LOOP sphere-steps."""
+    frames, device->host retirement.  Throughput counts LOOP sphere-steps.
+    The batched leg admits first come, first served; the most-steps-first
+    order is reported beside it."""
     from bevy_millions_ball_amd.serving import Server, poisson_requests, request_scene
 
     reqs = poisson_requests(n_requests, rate_per_s=200.0, seed=seed)
@@ -105,12 +107,23 @@ def serving_bench(n_requests: int, n_poisson: int, seed: int) -> dict:
         srv.drain()
         srv.collect(rid)
 
+    def batched(admission):
+        with Server(admission=admission) as srv:
+            ids = [srv.submit(st, pos, vf, r.steps) for (st, pos, vf), r in zip(scenes, reqs)]
+            t0 = time.perf_counter()
+            srv.drain()
+            el = time.perf_counter() - t0
+            frames = srv.stats()["frames"]
+            for i in ids:
+                srv.collect(i)
+        return el, frames
+
+    batched_s, frames_batched = batched("fifo")
+    lpt_s, frames_lpt = batched("most_steps_first")
+    # the same batch again with events around every frame, for device time
     with Server(profile_frames=True) as srv:
         ids = [srv.submit(st, pos, vf, r.steps) for (st, pos, vf), r in zip(scenes, reqs)]
-        t0 = time.perf_counter()
         srv.drain()
-        batched_s = time.perf_counter() - t0
-        frames_batched = srv.stats()["frames"]
         batched_device_s = float(srv.frame_times().sum()) * 1e-3
         for i in ids:
             srv.collect(i)
@@ -157,6 +170,7 @@ def serving_bench(n_requests: int, n_poisson: int, seed: int) -> dict:
         "requests": n_requests,
         "spheres_total": int(sum(r.number_of_agents for r in reqs)),
         "sphere_steps": work,
+        "admission": "fifo",
         "batched_sphere_steps_per_s": work / batched_s,
         "one_by_one_sphere_steps_per_s": work / one_s,
         "batched_over_one_by_one": one_s / batched_s,
@@ -164,6 +178,8 @@ def serving_bench(n_requests: int, n_poisson: int, seed: int) -> dict:
         "batched_frames_device_s": batched_device_s,
         "one_by_one_s": one_s,
         "frames_batched": int(frames_batched),
+        "most_steps_first": {"batched_s": lpt_s, "frames": int(frames_lpt),
+                             "batched_over_one_by_one": one_s / lpt_s},
         "frames_one_by_one": int(frames_one),
         "poisson": {
             "rate_per_s": 200.0,
@@ -187,7 +203,7 @@ def main() -> int:
     ap.add_argument("--config", default="uniform_1m")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--latency-steps", type=int, default=200)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=64)
     ap.add_argument("--poisson-requests", type=int, default=200)

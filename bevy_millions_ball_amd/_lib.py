@@ -58,7 +58,13 @@ class MbServerConfig(C.Structure):
         ("chunk_bytes", C.c_uint64),
         ("use_graph", C.c_int32),
         ("profile_frames", C.c_int32),
+        ("admission", C.c_int32),
+        ("reserved", C.c_int32),
     ]
+
+
+MB_ADMIT_FIFO = 0
+MB_ADMIT_MOST_STEPS_FIRST = 1
 
 
 class MbRequestStatus(C.Structure):

@@ -496,20 +496,51 @@ __device__ __forceinline__ uint2 vel_mass_of(uint4 q)
     return make_uint2(q.x, (q.y & 0xFFFFu) | (q.w << 16));
 }
 
+// t / d for three numerators that share the divisor, each bit-identical to
+// the IEEE quotient.  The general expansion scales its operands (div_scale)
+// only near the ends of the exponent range; when d lies in [2^-60, 2^60] and
+// every numerator is 0 or at least 2^-60 in magnitude (here |t| <= d), that
+// scaling is the identity, so one refined reciprocal serves all three
+// quotients, each with the expansion's two residual corrections.  The
+// residual is formed negated (d q - t) so that a zero numerator keeps its
+// sign.  Other operands take the general division.
+__device__ __forceinline__ void div3(float tx, float ty, float tz, float d, float &qx, float &qy,
+                                     float &qz)
+{
+    const int e = min(min(__builtin_amdgcn_frexp_expf(tx), __builtin_amdgcn_frexp_expf(ty)),
+                      __builtin_amdgcn_frexp_expf(tz));  // 0 for a zero numerator
+    if (e >= -59 && d >= 0x1p-60f && d <= 0x1p60f) {
+        const float y0 = __builtin_amdgcn_rcpf(d);
+        const float y1 = __builtin_fmaf(__builtin_fmaf(-d, y0, 1.0f), y0, y0);
+        auto q = [&](float t) {
+            const float q0 = t * y1;
+            const float q1 = __builtin_fmaf(-__builtin_fmaf(d, q0, -t), y1, q0);
+            return __builtin_fmaf(-__builtin_fmaf(d, q1, -t), y1, q1);
+        };
+        qx = q(tx);
+        qy = q(ty);
+        qz = q(tz);
+        return;
+    }
+    qx = tx / d;
+    qy = ty / d;
+    qz = tz / d;
+}
+
 // solve_sphere_constraint (physics.wgsl:300-331) from the offset
 // t = A.position - B.position, d2 = dot(t, t) and edge = r_B + r_A.
 // `vel_mass()` yields the neighbour's velocity and mass; it is called only
-// for a contact with the rigid response.
+// for a contact with the rigid response.  Returns whether the spheres touch
+// (only then can the sphere's position change).
 template <bool SPRING, typename VelMass>
-__device__ __forceinline__ void sphere_contact(const PhysConsts &c, Agent &a, float edge,
+__device__ __forceinline__ bool sphere_contact(const PhysConsts &c, Agent &a, float edge,
                                                float tx, float ty, float tz, float d2,
                                                VelMass vel_mass)
 {
     const float dist = sqrtf(d2);
     if (dist <= edge) {
-        const float nx = tx / dist;
-        const float ny = ty / dist;
-        const float nz = tz / dist;
+        float nx, ny, nz;
+        div3(tx, ty, tz, dist, nx, ny, nz);  // normalize(t): t / length(t)
         if (!SPRING) {
             const uint2 q = vel_mass();
             const float bvx = half_lo(q.x), bvy = half_hi(q.x), bvz = half_lo(q.y);
@@ -555,7 +586,9 @@ __device__ __forceinline__ void sphere_contact(const PhysConsts &c, Agent &a, fl
                 a.pz = a.pz + limit * nz;
             }
         }
+        return true;
     }
+    return false;
 }
 
 template <bool SPRING>
@@ -638,9 +671,18 @@ __device__ __forceinline__ void solve_cube(const PhysConsts &c, Agent &a)
     const bool ix = mx < 0.0f, iy = my < 0.0f, iz = mz < 0.0f;
     if (!(ix || iy || iz)) return;
     if (!SPRING) {
-        a.px = clamp_sel(tx, -ex, ex) + 0.0f;
-        a.py = clamp_sel(ty, -ey, ey) + c.half_map_height;
-        a.pz = clamp_sel(tz, -ez, ez) + 0.0f;
+        if (ex > 0.0f && ey > 0.0f && ez > 0.0f) {
+            // with bounds -e < 0 < e, v_max/v_min give exactly the
+            // compare-and-select clamp: a tie is between equal non-zero
+            // values, and a NaN operand yields the bound either way
+            a.px = fminf(fmaxf(tx, -ex), ex) + 0.0f;
+            a.py = fminf(fmaxf(ty, -ey), ey) + c.half_map_height;
+            a.pz = fminf(fmaxf(tz, -ez), ez) + 0.0f;
+        } else {
+            a.px = clamp_sel(tx, -ex, ex) + 0.0f;
+            a.py = clamp_sel(ty, -ey, ey) + c.half_map_height;
+            a.pz = clamp_sel(tz, -ez, ez) + 0.0f;
+        }
         a.vx = a.vx * (ix ? -c.e_envir : 1.0f);
         a.vy = a.vy * (iy ? -c.e_envir : 1.0f);
         a.vz = a.vz * (iz ? -c.e_envir : 1.0f);
@@ -1143,7 +1185,7 @@ static size_t row_sort_lds(uint32_t grids)
 // margin used here, so every candidate that can be in contact reaches the
 // exact test.
 template <bool SPRING, typename VelMass>
-__device__ __forceinline__ void collide_candidate(const PhysConsts &c, Agent &a, float4 bp,
+__device__ __forceinline__ bool collide_candidate(const PhysConsts &c, Agent &a, float4 bp,
                                                   VelMass vel_mass)
 {
     const float edge = bp.w + a.scale;
@@ -1151,7 +1193,9 @@ __device__ __forceinline__ void collide_candidate(const PhysConsts &c, Agent &a,
     const float ty = a.py - bp.y;
     const float tz = a.pz - bp.z;
     const float d2 = tx * tx + ty * ty + tz * tz;
-    if (d2 <= (edge * edge) * 1.0001f) sphere_contact<SPRING>(c, a, edge, tx, ty, tz, d2, vel_mass);
+    if (d2 <= (edge * edge) * 1.0001f)
+        return sphere_contact<SPRING>(c, a, edge, tx, ty, tz, d2, vel_mass);
+    return false;
 }
 
 // Walks the candidates of one sphere: middle, bottom and top ranges in the
@@ -1260,17 +1304,21 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
             const uint32_t lz = (uint32_t)__builtin_clz(near);
             const uint32_t t = end - 32u + lz;
             near ^= 0x80000000u >> lz;
+            bool touch;
             if (FROM_LDS) {
                 const uint32_t l = lds_of(t);
-                collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
+                touch = collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
             } else {
                 const uint32_t bo = perm[slot_of(t)];
-                collide_candidate<SPRING>(c, a, pos_i[bo], [&]() { return vel_mass_of(vfme_i[bo]); });
+                touch = collide_candidate<SPRING>(c, a, pos_i[bo],
+                                                  [&]() { return vel_mass_of(vfme_i[bo]); });
             }
-            const float ex = a.px - rx, ey = a.py - ry, ez = a.pz - rz;
-            if (!(ex * ex + ey * ey + ez * ez <= still)) {
-                next = t + 1u;  // moved beyond the margin: rescan after t
-                near = 0;
+            if (touch) {  // only a contact moves the sphere
+                const float ex = a.px - rx, ey = a.py - ry, ez = a.pz - rz;
+                if (!(ex * ex + ey * ey + ez * ez <= still)) {
+                    next = t + 1u;  // moved beyond the margin: rescan after t
+                    near = 0;
+                }
             }
         }
         j = next;
@@ -1343,15 +1391,18 @@ __device__ __forceinline__ void physics_step_body(
     }
     // the three windows share one pool, back to back
     const uint32_t wbase[3] = {0u, wcnt[0], wcnt[0] + wcnt[1]};
-    const bool staged = wcnt[0] + wcnt[1] + wcnt[2] <= POOL;
+    const uint32_t wtot = wcnt[0] + wcnt[1] + wcnt[2];
+    const bool staged = wtot <= POOL;
     if (staged) {
-#pragma unroll
-        for (uint32_t w = 0; w < 3; ++w) {
-            for (uint32_t j = tid; j < wcnt[w]; j += 256u) {
-                const uint32_t bo = perm[wlo[w] + j];
-                s_pos[wbase[w] + j] = pos_i[bo];
-                if (!SPRING) s_vm[wbase[w] + j] = vel_mass_of(vfme_i[bo]);
-            }
+        // one pass over the pool: index j of window w holds slot
+        // wlo[w] + j - wbase[w] (u32 arithmetic, wrapping)
+        const uint32_t d1 = (wlo[1] - wbase[1]) - wlo[0];
+        const uint32_t d2 = (wlo[2] - wbase[2]) - (wlo[1] - wbase[1]);
+        for (uint32_t j = tid; j < wtot; j += 256u) {
+            const uint32_t bo =
+                perm[j + wlo[0] + (j >= wbase[1] ? d1 : 0u) + (j >= wbase[2] ? d2 : 0u)];
+            s_pos[j] = pos_i[bo];
+            if (!SPRING) s_vm[j] = vel_mass_of(vfme_i[bo]);
         }
     }
     __syncthreads();

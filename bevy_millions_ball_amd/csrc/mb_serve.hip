@@ -416,13 +416,18 @@ int admit(mb_server *s)
     uint64_t in_flight = 0;
     for (const Request *r : s->uploading) in_flight += (uint64_t)r->n * 32;
     while (!s->queue.empty() && (in_flight < kUploadBytes || s->uploading.empty())) {
-        Request *r = s->queue.front();
+        auto next = s->queue.begin();
+        if (s->config.admission == MB_ADMIT_MOST_STEPS_FIRST) {
+            for (auto it = s->queue.begin(); it != s->queue.end(); ++it)
+                if ((*it)->steps > (*next)->steps) next = it;
+        }
+        Request *r = *next;
         if (s->used + r->layout.bytes > s->config.memory_cap_bytes) break;
         char *region = s->arena->alloc(r->layout.bytes);
         if (!region) break;  // the device is full: wait for retirements
         r->region = region;
         s->used += r->layout.bytes;
-        s->queue.pop_front();
+        s->queue.erase(next);
         const mb::SceneDesc d = r->desc();
         const size_t n = r->n;
         const char *in = static_cast<const char *>(r->io);
@@ -661,6 +666,9 @@ int mb_server_create(const mb_server_config *config, mb_server **out)
     *out = nullptr;
     if (config->memory_cap_bytes == 0)
         return fail(MB_ERR_INVALID_ARGUMENT, "memory_cap_bytes must be > 0");
+    if (config->admission != MB_ADMIT_FIFO && config->admission != MB_ADMIT_MOST_STEPS_FIRST)
+        return fail(MB_ERR_INVALID_ARGUMENT, "admission must be MB_ADMIT_FIFO or MB_ADMIT_MOST_STEPS_FIRST");
+    if (config->reserved != 0) return fail(MB_ERR_INVALID_ARGUMENT, "reserved must be 0");
     mb_server *s = new (std::nothrow) mb_server();
     if (!s) return fail(MB_ERR_OUT_OF_MEMORY, "host allocation failed");
     s->config = *config;

@@ -25,6 +25,8 @@ from typing import List, Optional
 import numpy as np
 
 from ._lib import (
+    MB_ADMIT_FIFO,
+    MB_ADMIT_MOST_STEPS_FIRST,
     MB_REQUEST_DONE,
     MB_REQUEST_QUEUED,
     MB_REQUEST_RETIRING,
@@ -54,14 +56,26 @@ class Collected:
     next_keys: Optional[np.ndarray] = None
 
 
+ADMISSION = {"fifo": MB_ADMIT_FIFO, "most_steps_first": MB_ADMIT_MOST_STEPS_FIRST}
+
+
 class Server:
+    """mb_server_*: independent scenes stepped together in batched frames.
+
+    admission: "fifo" (first come, first served) or "most_steps_first" (the
+    queued request with the most LOOP frames goes next)."""
+
     def __init__(self, memory_cap_bytes: int = 200 * 10**9, chunk_bytes: int = 0,
-                 use_graph: bool = False, profile_frames: bool = False):
+                 use_graph: bool = False, profile_frames: bool = False,
+                 admission: str = "fifo"):
+        if admission not in ADMISSION:
+            raise ValueError(f"admission must be one of {sorted(ADMISSION)}")
         cfg = MbServerConfig()
         cfg.memory_cap_bytes = int(memory_cap_bytes)
         cfg.chunk_bytes = int(chunk_bytes)
         cfg.use_graph = int(bool(use_graph))
         cfg.profile_frames = int(bool(profile_frames))
+        cfg.admission = ADMISSION[admission]
         self._h = C.c_void_p()
         check(lib().mb_server_create(C.byref(cfg), C.byref(self._h)))
         self._n = {}

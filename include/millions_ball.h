@@ -236,7 +236,16 @@ typedef struct mb_server_config {
                                   (0: 4 GiB; at most the cap) */
     int32_t use_graph;         /* replay each batch composition as a captured hipGraph */
     int32_t profile_frames;    /* record each frame's device time (HIP events) */
+    int32_t admission;         /* order in which queued requests are admitted: MB_ADMIT_* */
+    int32_t reserved;          /* 0 */
 } mb_server_config;
+
+/* Admission orders.  FIFO: first come, first served.  MOST_STEPS_FIRST: the
+ * queued request with the most LOOP frames (the earliest of equals) goes
+ * next, which shortens the time until a batch submitted at once has finished:
+ * long requests start first and short ones fill in behind them. */
+#define MB_ADMIT_FIFO 0
+#define MB_ADMIT_MOST_STEPS_FIRST 1
 
 #define MB_REQUEST_QUEUED 0
 #define MB_REQUEST_RUNNING 1   /* admitted: its upload or its frames are under way */

@@ -76,6 +76,32 @@ def test_alone_equals_mix(gpu, use_graph):
                                         for st, _, _, steps in MIX)
 
 
+def test_most_steps_first_admission_keeps_results(gpu):
+    # admission order changes when a scene starts, never what it computes;
+    # a cap that holds few scenes at a time makes the order matter
+    alone = run_alone(MIX)
+    cap = 3 * max(Server.scene_bytes(st.max_number_of_agents(), st.number_of_grids_one_dimension)
+                  for st, _, _, _ in MIX)
+    mixed, stats = run_mix(MIX, admission="most_steps_first", memory_cap_bytes=cap)
+    for i, (a, b) in enumerate(zip(alone, mixed)):
+        assert_same(a, b, f"scene {i} (N={MIX[i][0].max_number_of_agents()})")
+    assert stats["sphere_steps"] == sum(st.max_number_of_agents() * steps
+                                        for st, _, _, steps in MIX)
+
+
+def test_admission_order_is_validated(gpu):
+    with pytest.raises(ValueError):
+        Server(admission="shortest")
+    from bevy_millions_ball_amd import _lib
+    import ctypes as C
+    cfg = _lib.MbServerConfig()
+    cfg.memory_cap_bytes = 1 << 30
+    cfg.admission = 7
+    h = C.c_void_p()
+    rc = _lib.lib().mb_server_create(C.byref(cfg), C.byref(h))
+    assert rc == _lib.MB_ERR_INVALID_ARGUMENT and not h.value
+
+
 def test_mix_matches_single_scene_physics(gpu):
     mixed, _ = run_mix(MIX)
     checked = 0
