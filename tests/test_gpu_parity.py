@@ -287,6 +287,38 @@ def test_edge_cases_tiny_worlds(gpu, oracle):
             compare_frames(g, r, 8)
 
 
+def test_exact_rewrite_fallback_paths(gpu, oracle):
+    """The kernels' exact rewrites next to their general code, in one wave:
+    contacts whose offset has a component below 2^-60 (general division),
+    an offset with a signed zero (shared reciprocal), and radii at and above
+    the half map size (compare-and-select wall clamp with -e >= e)."""
+    rng = np.random.default_rng(21)
+    n, grids = 48, 4
+    ps = PhysicsSettings(number_of_grids_one_dimension=grids, grid_size=1, half_map_height=4,
+                         e_agent=0.8, e_envir=0.8, time_delta=1 / 60,
+                         _max_number_of_agents=n)
+    pos = np.empty((n, 4), np.float32)
+    pos[:, 0] = rng.uniform(-1.5, 1.5, n)
+    pos[:, 2] = rng.uniform(-1.5, 1.5, n)
+    pos[:, 1] = rng.uniform(0.5, 7.5, n)
+    pos[:, 3] = 0.5
+    pos[0] = (1e-30, 2.0, 0.25, 0.5)     # against sphere 1: |t.x| = 1e-30 < 2^-60
+    pos[1] = (0.0, 2.6, 0.25, 0.5)
+    pos[2] = (-0.0, 5.0, -1.0, 0.5)      # against sphere 3: t.x = -0
+    pos[3] = (0.0, 5.5, -1.2, 0.5)
+    pos[4] = (0.3, 3.0, 0.3, 2.5)        # radius above the half map size (2)
+    pos[5] = (-0.7, 6.0, 0.9, 2.0)       # radius equal to it: bounds of zero
+    vel = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    vel[:6] = 0.0
+    vf = pack_vfme(vel, np.zeros((n, 3), np.float32), np.ones(n, np.float32))
+    g = Physics(ps)
+    g.upload_state(pos, vf)
+    r = oracle.World(ps)
+    r.upload(pos, vf)
+    with g:
+        compare_frames(g, r, 6)
+
+
 def test_dense_rows_spill_and_fallbacks(gpu, oracle):
     """Rows far above the mean: bucket spill, the global-memory row sort and
     the collide path that reads neighbours from global memory."""
