@@ -195,6 +195,40 @@ def serving_bench(n_requests: int, n_poisson: int, seed: int) -> dict:
     }
 
 
+def other_config(name: str, seed: int, steps: int) -> dict:
+    """Throughput and per-kernel device time of another workload (the
+    parity-test configurations), `steps` LOOP frames after INIT + 10."""
+    import torch
+
+    from bevy_millions_ball_amd import CONFIGS, Physics, generate
+
+    cfg = CONFIGS[name]
+    n = cfg.number_of_agents
+    cells = cfg.grids * cfg.grids
+    pos, vf = generate(cfg, seed=seed)
+    with Physics(cfg.settings()) as phys:
+        phys.upload_state(pos, vf)
+        phys.run(11)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        phys.run(steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        phys.set_profiling(True)
+        phys.run(steps)
+        torch.cuda.synchronize()
+        stage_ms = phys.profile_read().mean(axis=0)
+        phys.set_profiling(False)
+    achieved = physics_bytes_per_sphere(n, cells) * n / (stage_ms[1] * 1e-3) / 1e9
+    return {
+        "spheres": n, "grids": cfg.grids, "spring": cfg.enable_spring, "layout": cfg.layout,
+        "time_delta": cfg.time_delta, "steps": steps,
+        "value": n * steps / el, "unit": "sphere-steps/s", "ms_per_step": el * 1e3 / steps,
+        "stage_ms": {"row_sort": float(stage_ms[0]), "collide_integrate": float(stage_ms[1])},
+        "collide_roofline_frac": achieved / HBM_PEAK_GBS,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +242,7 @@ def main() -> int:
     ap.add_argument("--serving-requests", type=int, default=64)
     ap.add_argument("--poisson-requests", type=int, default=200)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-other-configs", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -314,6 +349,9 @@ def main() -> int:
             "bytes_per_launch": phys_bytes,
         },
     }
+    if not args.no_other_configs and args.config == "uniform_1m":
+        line["other_configs"] = {name: other_config(name, args.seed, steps) for name, steps in (
+            ("plain_65k", 500), ("spring_clumps_1m", 100), ("uniform_4m", 100))}
     if not args.no_serving:
         line["serving"] = serving_bench(args.serving_requests, args.poisson_requests, args.seed)
     if not args.no_cpu_baseline:

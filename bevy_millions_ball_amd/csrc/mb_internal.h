@@ -207,6 +207,7 @@ struct BatchPlan {
     const uint2 *blk_spring;  // blocks of 256 sorted slots, spring on
     uint32_t blk_spring_count;
     uint32_t max_grids;       // largest G among the scenes (row sort LDS)
+    uint32_t max_row_mean;    // largest ceil(N / G) among the scenes (row sort width)
 };
 
 // Enqueues frame k of the plan for every scene (INIT for those at time 0).

@@ -1175,9 +1175,10 @@ __global__ __launch_bounds__(kSortThreads) void row_sort_kernel(
                                 sorted_keys, xcd_block(blockIdx.x, gridDim.x));
 }
 
+template <uint32_t NT = kSortThreads>
 static size_t row_sort_lds(uint32_t grids)
 {
-    return ((size_t)row_sort_cells(grids, kSortThreads) + kRowCap) * 4u;
+    return ((size_t)row_sort_cells(grids, NT) + kRowCap) * 4u;
 }
 
 // solve_sphere_constraint behind a cheap conservative filter:
@@ -1589,10 +1590,11 @@ __device__ __forceinline__ uint32_t plan_frame(uint32_t k, const uint32_t *count
     return counter ? *counter : k;
 }
 
-__global__ __launch_bounds__(kSortThreads) void batch_row_sort_kernel(const SceneDesc *__restrict__ scenes,
-                                                             const uint2 *__restrict__ map,
-                                                             uint32_t k,
-                                                             const uint32_t *__restrict__ counter)
+template <uint32_t NT>
+__global__ __launch_bounds__(NT) void batch_row_sort_kernel(const SceneDesc *__restrict__ scenes,
+                                                            const uint2 *__restrict__ map,
+                                                            uint32_t k,
+                                                            const uint32_t *__restrict__ counter)
 {
     const uint2 m = map[xcd_block(blockIdx.x, gridDim.x)];
     const SceneDesc &S = scenes[m.x];
@@ -1603,7 +1605,7 @@ __global__ __launch_bounds__(kSortThreads) void batch_row_sort_kernel(const Scen
     const RowBuckets rb = scene_buckets(S);
     if (time == 0) {
         // INIT: cell 0 holds every sphere; clear bucket set 1 for the INIT step
-        for (uint32_t c = threadIdx.x; c < G; c += kSortThreads)
+        for (uint32_t c = threadIdx.x; c < G; c += NT)
             off[m.y * G + c] = (m.y == 0u && c == 0u) ? 0u : S.n;
         if (threadIdx.x == 0) {
             if (m.y == G - 1u) off[G * G] = S.n;
@@ -1612,8 +1614,8 @@ __global__ __launch_bounds__(kSortThreads) void batch_row_sort_kernel(const Scen
         }
         return;
     }
-    row_sort_body<kSortThreads>(rb, (uint32_t)time & 1u, gptr(S.scratch_pairs), gptr(S.scratch_idx), G, S.n,
-                  gptr(S.perm), off, gptr(S.keys), m.y);
+    row_sort_body<NT>(rb, (uint32_t)time & 1u, gptr(S.scratch_pairs), gptr(S.scratch_idx), G, S.n,
+                      gptr(S.perm), off, gptr(S.keys), m.y);
 }
 
 template <bool SPRING>
@@ -1669,9 +1671,16 @@ hipError_t launch_batch_frame(const BatchPlan &p, uint32_t k, bool from_counter,
     const uint32_t *counter = from_counter ? p.frame_counter : nullptr;
     if (p.scene_count == 0) return hipSuccess;
     if (p.rows) {
-        hipLaunchKernelGGL(batch_row_sort_kernel, dim3(p.rows), dim3(kSortThreads),
-                           row_sort_lds(p.max_grids), stream,
-                           p.scenes, p.row_map, k, counter);
+        // rows of at most 1024 spheres on average: 256-thread blocks, more
+        // of them resident at once (each row costs about the same latency);
+        // denser rows: kSortThreads, more threads per row
+        if (p.max_row_mean <= 1024u)
+            hipLaunchKernelGGL(batch_row_sort_kernel<256>, dim3(p.rows), dim3(256),
+                               row_sort_lds<256>(p.max_grids), stream, p.scenes, p.row_map, k,
+                               counter);
+        else
+            hipLaunchKernelGGL(batch_row_sort_kernel<kSortThreads>, dim3(p.rows), dim3(kSortThreads),
+                               row_sort_lds(p.max_grids), stream, p.scenes, p.row_map, k, counter);
     }
     if (p.blk_plain_count)
         hipLaunchKernelGGL(batch_physics_step_kernel<false>, dim3(p.blk_plain_count), dim3(256), 0,

@@ -470,11 +470,12 @@ int upload_plan(mb_server *s)
     std::vector<mb::SceneDesc> descs;
     std::vector<uint2> rows, plain, spring;
     descs.reserve(count);
-    uint32_t max_g = 1;
+    uint32_t max_g = 1, max_rm = 1;
     for (uint32_t i = 0; i < count; ++i) {
         const Request *r = s->running[i];
         descs.push_back(r->desc());
         max_g = std::max(max_g, r->g);
+        max_rm = std::max(max_rm, (r->n + r->g - 1u) / r->g);
         for (uint32_t g = 0; g < r->g; ++g) rows.push_back(make_uint2(i, g));
         auto &blk = r->consts.enable_spring ? spring : plain;
         for (uint32_t b = 0; b < (r->n + 255u) / 256u; ++b) blk.push_back(make_uint2(i, b));
@@ -532,6 +533,7 @@ int upload_plan(mb_server *s)
     p.blk_spring = p.blk_plain + plain.size();
     p.blk_spring_count = (uint32_t)spring.size();
     p.max_grids = max_g;
+    p.max_row_mean = max_rm;
 
     if (s->config.use_graph) {
         if (s->exec) {

@@ -76,6 +76,29 @@ def test_alone_equals_mix(gpu, use_graph):
                                         for st, _, _, steps in MIX)
 
 
+# a scene of ~2200 spheres per row: the batched row sort takes its wide blocks
+DENSE_MIX = [
+    scene(70000, 3, 11, grids=32),
+    scene(4096, 4, 12),
+    scene(20000, 2, 13, enable_spring=True, k_agent=500.0, c_agent=5.0, time_delta=1.0 / 120.0),
+]
+
+
+def test_dense_rows_mix(gpu, oracle):
+    alone = run_alone(DENSE_MIX)
+    mixed, _ = run_mix(DENSE_MIX)
+    for i, (a, b) in enumerate(zip(alone, mixed)):
+        assert_same(a, b, f"scene {i}")
+    for (st, pos, vf, steps), got in zip(DENSE_MIX, mixed):
+        w = oracle.World(st)
+        w.upload(pos, vf)
+        w.step(1 + steps)
+        p2, v2 = w.download()
+        w.close()
+        assert np.array_equal(p2.view(np.uint32), got.position_with_scale.view(np.uint32))
+        assert np.array_equal(v2, got.packed_vfme)
+
+
 def test_most_steps_first_admission_keeps_results(gpu):
     # admission order changes when a scene starts, never what it computes;
     # a cap that holds few scenes at a time makes the order matter
