@@ -37,7 +37,8 @@
 namespace {
 
 constexpr uint64_t kAlign = 256;
-constexpr uint32_t kDepth = 2;  // frames in flight before a tick waits
+constexpr uint32_t kDepth = 2;       // throttle events in flight before a tick waits
+constexpr uint32_t kEventEvery = 4;  // frames per throttle event (so <= 8 frames ahead)
 
 uint64_t align_up(uint64_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
 
@@ -750,7 +751,8 @@ int mb_server_tick(mb_server *s, uint32_t *running)
     rc = join_uploaded(s);
     if (rc != MB_OK) return rc;
     if (s->running.empty()) return MB_OK;
-    // keep at most kDepth frames in flight so that the host does not run ahead
+    // keep at most kDepth throttle events (kDepth * kEventEvery frames) in
+    // flight so that the host does not run far ahead
     while (s->inflight.size() >= kDepth) {
         MB_HIP(hipEventSynchronize(s->inflight.front()));
         s->spare_events.push_back(s->inflight.front());
@@ -786,10 +788,14 @@ int mb_server_tick(mb_server *s, uint32_t *running)
         MB_HIP(hipEventRecord(t1, s->compute));
         s->timed.emplace_back(t0, t1);
     }
-    hipEvent_t fe = s->take_event();
-    if (!fe) return fail(MB_ERR_HIP, "event creation failed");
-    MB_HIP(hipEventRecord(fe, s->compute));
-    s->inflight.push_back(fe);
+    // the run-ahead throttle's event, on every kEventEvery-th frame only: an
+    // event between two frames holds the next one back by ~10 µs
+    if (s->frames % kEventEvery == 0) {
+        hipEvent_t fe = s->take_event();
+        if (!fe) return fail(MB_ERR_HIP, "event creation failed");
+        MB_HIP(hipEventRecord(fe, s->compute));
+        s->inflight.push_back(fe);
+    }
     ++s->frames;
     ++s->plan_k;
     if (running) *running = (uint32_t)s->running.size();
