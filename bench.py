@@ -195,6 +195,34 @@ def serving_bench(n_requests: int, n_poisson: int, seed: int) -> dict:
     }
 
 
+def serving_trace(n_requests: int, seed: int) -> int:
+    """One batched drain (first come, first served) of the serving mix, after
+    a warm-up server; prints its wall time.  Made to run under a kernel trace."""
+    import torch
+
+    from bevy_millions_ball_amd.serving import Server, poisson_requests, request_scene
+
+    torch.cuda.set_device(0)
+    reqs = poisson_requests(n_requests, rate_per_s=200.0, seed=seed)
+    scenes = [request_scene(r) for r in reqs]
+    with Server() as srv:
+        st, pos, vf = scenes[0]
+        rid = srv.submit(st, pos, vf, 2)
+        srv.drain()
+        srv.collect(rid)
+    with Server() as srv:
+        ids = [srv.submit(st, pos, vf, r.steps) for (st, pos, vf), r in zip(scenes, reqs)]
+        t0 = time.perf_counter()
+        srv.drain()
+        el = time.perf_counter() - t0
+        frames = srv.stats()["frames"]
+        for i in ids:
+            srv.collect(i)
+    print(json.dumps({"serving_trace": {"requests": n_requests, "batched_s": el, "frames": int(frames)}}),
+          flush=True)
+    return 0
+
+
 def other_config(name: str, seed: int, steps: int) -> dict:
     """Throughput and per-kernel device time of another workload (the
     parity-test configurations), `steps` LOOP frames after INIT + 10."""
@@ -243,6 +271,8 @@ def main() -> int:
     ap.add_argument("--poisson-requests", type=int, default=200)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-other-configs", action="store_true")
+    ap.add_argument("--serving-trace", action="store_true",
+                    help="only one batched drain of the serving mix (for a kernel trace)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -253,6 +283,9 @@ def main() -> int:
     import torch
 
     from bevy_millions_ball_amd import CONFIGS, Physics, generate
+
+    if args.serving_trace:
+        return serving_trace(args.serving_requests, args.seed)
 
     cfg = CONFIGS[args.config]
     n = cfg.number_of_agents
