@@ -225,7 +225,8 @@ def serving_trace(n_requests: int, seed: int) -> int:
 
 def other_config(name: str, seed: int, steps: int) -> dict:
     """Throughput and per-kernel device time of another workload (the
-    parity-test configurations), `steps` LOOP frames after INIT + 10."""
+    parity-test configurations), `steps` LOOP frames after INIT + 10 and the
+    autotuner's trials."""
     import torch
 
     from bevy_millions_ball_amd import CONFIGS, Physics, generate
@@ -237,6 +238,8 @@ def other_config(name: str, seed: int, steps: int) -> dict:
     with Physics(cfg.settings()) as phys:
         phys.upload_state(pos, vf)
         phys.run(11)
+        # the library's autotuner picks the launch choices (never the results)
+        tuning, _ = phys.autotune(frames_per_trial=5)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         phys.run(steps)
@@ -251,6 +254,7 @@ def other_config(name: str, seed: int, steps: int) -> dict:
     return {
         "spheres": n, "grids": cfg.grids, "spring": cfg.enable_spring, "layout": cfg.layout,
         "time_delta": cfg.time_delta, "steps": steps,
+        "tuning": {"scan_margin": tuning[0], "stage_pool": tuning[1]},
         "value": n * steps / el, "unit": "sphere-steps/s", "ms_per_step": el * 1e3 / steps,
         "stage_ms": {"row_sort": float(stage_ms[0]), "collide_integrate": float(stage_ms[1])},
         "collide_roofline_frac": achieved / HBM_PEAK_GBS,
