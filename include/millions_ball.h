@@ -242,8 +242,8 @@ typedef struct mb_server_config {
 
 /* Admission orders.  FIFO: first come, first served.  MOST_STEPS_FIRST: the
  * queued request with the most LOOP frames (the earliest of equals) goes
- * next, which shortens the time until a batch submitted at once has finished:
- * long requests start first and short ones fill in behind them. */
+ * next, so a batch submitted at once runs in fewer, fuller frames; the long
+ * requests then end together, and their copies back to the host with them. */
 #define MB_ADMIT_FIFO 0
 #define MB_ADMIT_MOST_STEPS_FIRST 1
 
