@@ -78,7 +78,8 @@ def test_scene_bytes_covers_every_array():
     per_sphere = 16 * 4 + 4 * 3 + 8 + 8
     cap = max(4 * -(-n // g), 512)
     cap = -(-cap // 64) * 64
-    want = per_sphere * n + 8 * g * cap + 4 * (g * g + 1) + 4 * 32 * (2 * g + 2)
+    # buckets, cell offsets, and one packed word per fill / spill counter
+    want = per_sphere * n + 8 * g * cap + 4 * (g * g + 1) + 4 * (2 * g + 2)
     got = Server.scene_bytes(n, g)
     assert want <= got <= want + 12 * 256
 
