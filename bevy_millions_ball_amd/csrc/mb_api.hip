@@ -149,7 +149,8 @@ struct mb_physics {
     uint32_t *keys[2] = {nullptr, nullptr};
     uint32_t *vals[2] = {nullptr, nullptr};
     uint32_t *off = nullptr;        // per-cell slot offsets, N + 1
-    // row buckets: counters (2 sets of kMaxRows + the two spill counts), the
+    // row buckets: counters (the spread set and 2 packed sets of kMaxRows,
+    // the two spill counts), the
     // G x cap buckets (sized for the current grid) and the spill list
     uint32_t *counters = nullptr;
     uint64_t *bucket = nullptr;
@@ -173,8 +174,7 @@ struct mb_physics {
         mb::RowBuckets rb;
         rb.bucket = bucket;
         rb.spill = spill;
-        rb.fill = counters;  // set-major, G counters per set
-        rb.spill_count = counters + 2 * (size_t)mb::kMaxRows * mb::kLine;
+        mb::bucket_counters(counters, mb::kMaxRows, rb);
         rb.cap = bucket_cap;
         return rb;
     }

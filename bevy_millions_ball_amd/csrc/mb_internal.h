@@ -121,17 +121,37 @@ constexpr uint32_t kStagePoolBig = 1536; // the same for dense scenes (fewer blo
 // prefix over the counters of all lower rows reads 32 counters per line
 // (the frame at 1M spheres: 49.1 us one per 128-byte line, 46.9 us packed).
 constexpr uint32_t kLine = 1;
+// Filings of spheres in input order (the INIT step, a re-key) cannot be
+// aggregated per block: each wave files into ~64 rows with one atomic per
+// row.  They count in a third set, one counter per 128-byte line (kSpread
+// words apart), where packed counters would queue 32 rows' atomics on a line
+// (INIT at 1M spheres: 207 us packed, 83 us spread).  The row sort of frame 1
+// reads that set; a re-key copies it into the set its next frame reads.
+constexpr uint32_t kSpread = 32;
 
 struct RowBuckets {
     uint64_t *bucket;         // G x cap pairs (column << 32 | slot)
     uint64_t *spill;          // n entries
     uint32_t *fill;           // 2 x G counters, set-major, kLine words apart
     uint32_t *spill_count;    // 2 counters, kLine words apart
+    uint32_t *spread;         // G counters, kSpread words apart: filings in input order
     uint32_t cap;             // pairs per bucket
 };
 
-// Words of the counters of a G-row bucket set pair (both sets, both spill counts).
-inline size_t bucket_counter_words(uint32_t g) { return (2 * (size_t)g + 2) * kLine; }
+// Words of the counters of G rows: the spread set, both packed sets, both
+// spill counts, in that order.
+inline size_t bucket_counter_words(uint32_t g)
+{
+    return (size_t)g * kSpread + (2 * (size_t)g + 2) * kLine;
+}
+
+// Counter pointers of G rows laid out as bucket_counter_words describes.
+inline void bucket_counters(uint32_t *counters, uint32_t g, RowBuckets &rb)
+{
+    rb.spread = counters;
+    rb.fill = counters + (size_t)g * kSpread;
+    rb.spill_count = rb.fill + 2 * (size_t)g * kLine;
+}
 
 // Bucket capacity for n spheres over G rows: four times the mean row, at
 // least 512, a multiple of 64.
@@ -168,8 +188,9 @@ struct FrameTuning {
 hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time, uint32_t n,
                         hipStream_t stream, hipEvent_t *events, FrameTuning tuning = {});
 
-// Zeroes bucket set `set`, then writes key_out[i] = cell key of pos[i] and
-// files every sphere (slot i) in that set.
+// Writes key_out[i] = cell key of pos[i] and files every sphere (slot i),
+// counting in the (cleared) spread set, whose counts then replace bucket
+// set `set` and spill count `set`.
 hipError_t launch_rekey(const PhysConsts &c, const float4 *pos, uint32_t *key_out,
                         const RowBuckets &rb, uint32_t set, uint32_t n, hipStream_t stream);
 

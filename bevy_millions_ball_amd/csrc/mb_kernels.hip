@@ -942,6 +942,7 @@ __device__ __forceinline__ uint32_t block_scan_lds(uint32_t *vals, uint32_t coun
 // share a row take consecutive places with one returning atomic per distinct
 // row, the leaders of all rows issuing theirs at once.  A pair beyond the
 // bucket's capacity goes to the spill list.  Every active lane files one pair.
+template <uint32_t STRIDE>
 __device__ __forceinline__ void bucket_file(const RowBuckets &rb, uint32_t *__restrict__ fill,
                                             uint32_t *__restrict__ spill_count, uint32_t row,
                                             uint32_t col, uint32_t slot)
@@ -962,7 +963,7 @@ __device__ __forceinline__ void bucket_file(const RowBuckets &rb, uint32_t *__re
         todo &= ~peers;
     }
     uint32_t first = 0;
-    if (count) first = atomicAdd(&fill[row * kLine], count);
+    if (count) first = atomicAdd(&fill[row * STRIDE], count);
     const uint32_t at = (uint32_t)__shfl((int)first, (int)leader_lane, 64) + rank;
     if (at < rb.cap) {
         rb.bucket[(size_t)row * rb.cap + at] = ((uint64_t)col << 32) | slot;
@@ -1002,7 +1003,7 @@ __device__ __forceinline__ void block_file(BlockFiling &f, const RowBuckets &rb,
         const uint32_t rank = atomicAdd(&f.count[w], 1u);
         entry = (w << 24) | (rank << 12) | col;
     } else {
-        bucket_file(rb, fill, spill_count, row, col, s0 + tid);
+        bucket_file<kLine>(rb, fill, spill_count, row, col, s0 + tid);
     }
     f.entry[tid] = entry;
     __threadfence_block();
@@ -1045,8 +1046,9 @@ __host__ __device__ constexpr uint32_t row_sort_cells(uint32_t grids, uint32_t n
 }
 
 // One block per row r of frame `time` (>= 1).  The row's pairs are in bucket
-// set time & 1 (spilled pairs in the spill list); its slots start after the
-// spheres of all lower rows.  Counting sort by column in LDS, then each index
+// set time & 1 (spilled pairs in the spill list), counted in that set or, for
+// frame 1 (`spread_in`), in the spread set the INIT step filled; its slots
+// start after the spheres of all lower rows.  Counting sort by column in LDS, then each index
 // is ranked among the indices of its cell, which makes the result exactly
 // the stable argsort of the keys.  Writes perm, the sorted keys and the
 // per-cell offsets of the row, and clears bucket set !(time & 1) for the
@@ -1055,7 +1057,7 @@ __host__ __device__ constexpr uint32_t row_sort_cells(uint32_t grids, uint32_t n
 // indices.
 template <uint32_t NT>
 __device__ __forceinline__ void row_sort_body(
-    const RowBuckets &rb, uint32_t set, uint64_t *__restrict__ scratch_pairs,
+    const RowBuckets &rb, uint32_t set, bool spread_in, uint64_t *__restrict__ scratch_pairs,
     uint32_t *__restrict__ scratch_idx, uint32_t grids, uint32_t n,
     uint32_t *__restrict__ perm, uint32_t *__restrict__ off, uint32_t *__restrict__ sorted_keys,
     uint32_t r)
@@ -1066,12 +1068,13 @@ __device__ __forceinline__ void row_sort_body(
     __shared__ uint32_t s_scratch[NT / 64u];
     __shared__ uint32_t s_spilled;
     const uint32_t tid = threadIdx.x;
-    const uint32_t *fill_in = rb.fill + set * grids * kLine;
+    const uint32_t *fill_in = spread_in ? rb.spread : rb.fill + set * grids * kLine;
+    const uint32_t stride = spread_in ? kSpread : kLine;
     uint32_t part = 0;
-    for (uint32_t q = tid; q < r; q += NT) part += fill_in[q * kLine];
+    for (uint32_t q = tid; q < r; q += NT) part += fill_in[q * stride];
     uint32_t base;
     (void)block_exclusive_scan<NT>(part, s_scratch, base);  // base = spheres in rows < r
-    const uint32_t m = fill_in[r * kLine];
+    const uint32_t m = fill_in[r * stride];
     if (tid == 0) {
         rb.fill[((set ^ 1u) * grids + r) * kLine] = 0u;
         if (r == 0) rb.spill_count[(set ^ 1u) * kLine] = 0u;
@@ -1167,11 +1170,11 @@ __device__ __forceinline__ void row_sort_body(
 }
 
 __global__ __launch_bounds__(kSortThreads) void row_sort_kernel(
-    RowBuckets rb, uint32_t set, uint64_t *__restrict__ scratch_pairs,
+    RowBuckets rb, uint32_t set, uint32_t spread_in, uint64_t *__restrict__ scratch_pairs,
     uint32_t *__restrict__ scratch_idx, uint32_t grids, uint32_t n,
     uint32_t *__restrict__ perm, uint32_t *__restrict__ off, uint32_t *__restrict__ sorted_keys)
 {
-    row_sort_body<kSortThreads>(rb, set, scratch_pairs, scratch_idx, grids, n, perm, off,
+    row_sort_body<kSortThreads>(rb, set, spread_in != 0u, scratch_pairs, scratch_idx, grids, n, perm, off,
                                 sorted_keys, xcd_block(blockIdx.x, gridDim.x));
 }
 
@@ -1455,7 +1458,7 @@ __device__ __forceinline__ void physics_init_body(
     vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
                            pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
     key_out[s] = cz * c.grids + cx;
-    bucket_file(rb, rb.fill + c.grids * kLine, rb.spill_count + kLine, cz, cx, s);  // set 1: frame 1
+    bucket_file<kSpread>(rb, rb.spread, rb.spill_count + kLine, cz, cx, s);  // read by frame 1
 }
 
 __global__ __launch_bounds__(256) void physics_init_step_kernel(
@@ -1469,24 +1472,34 @@ __global__ __launch_bounds__(256) void physics_init_step_kernel(
 }
 
 // Offsets of the INIT frame: cell 0 holds every sphere.
-// Also clears bucket set 1, which the INIT step fills for frame 1.
+// Also clears the spread counters and spill count 1, which the INIT step
+// fills for frame 1.
 __global__ __launch_bounds__(256) void init_offsets_kernel(uint32_t *__restrict__ off, uint32_t cells,
                                                            uint32_t n, RowBuckets rb, uint32_t grids)
 {
     const uint32_t c = blockIdx.x * 256u + threadIdx.x;
     if (c <= cells) off[c] = c == 0 ? 0u : n;
-    if (c < grids) rb.fill[(grids + c) * kLine] = 0u;
+    if (c < grids) rb.spread[c * kSpread] = 0u;
     if (c == 0) rb.spill_count[kLine] = 0u;
 }
 
-// Clears bucket set `set` (first kernel) and files every sphere in it with
-// its key (second kernel).
+// A re-key: clears the spread counters and spill count `set` (first kernel),
+// files every sphere with its key, counting in the spread set (second), and
+// copies the counts into bucket set `set` (third).  Frame 1 reads the spread
+// set itself, any later frame its packed set.
 __global__ __launch_bounds__(256) void bucket_clear_kernel(RowBuckets rb, uint32_t set,
                                                            uint32_t grids)
 {
     const uint32_t r = blockIdx.x * 256u + threadIdx.x;
-    if (r < grids) rb.fill[(set * grids + r) * kLine] = 0u;
+    if (r < grids) rb.spread[r * kSpread] = 0u;
     if (r == 0) rb.spill_count[set * kLine] = 0u;
+}
+
+__global__ __launch_bounds__(256) void spread_pack_kernel(RowBuckets rb, uint32_t set,
+                                                          uint32_t grids)
+{
+    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    if (r < grids) rb.fill[(set * grids + r) * kLine] = rb.spread[r * kSpread];
 }
 
 __global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *__restrict__ pos,
@@ -1499,7 +1512,7 @@ __global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *
     uint32_t cx, cz;
     cell_of(c, p.x, p.z, cx, cz);
     key_out[s] = cz * c.grids + cx;
-    bucket_file(rb, rb.fill + set * c.grids * kLine, rb.spill_count + set * kLine, cz, cx, s);
+    bucket_file<kSpread>(rb, rb.spread, rb.spill_count + set * kLine, cz, cx, s);
 }
 
 template <uint32_t POOL>
@@ -1540,7 +1553,8 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time
     }
     const uint32_t set = (uint32_t)(time & 1);
     hipLaunchKernelGGL(row_sort_kernel, dim3(G), dim3(kSortThreads), row_sort_lds(G), stream, b.rb, set,
-                       b.scratch_pairs, b.scratch_idx, G, n, b.perm, b.off, b.keys_out);
+                       time == 1 ? 1u : 0u, b.scratch_pairs, b.scratch_idx, G, n, b.perm, b.off,
+                       b.keys_out);
     if ((e = mark(1)) != hipSuccess) return e;
     if (tuning.big_pool)
         launch_collide<kStagePoolBig>(b, c, set ^ 1u, n, stream);
@@ -1555,7 +1569,8 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time
 // list and runs the single-scene body on that scene's arrays, so a scene's
 // results are the same alone and in any mix.  A scene at time 0 runs the
 // INIT frame: its row sort blocks write its INIT offsets and clear its
-// bucket set 1, and its collide blocks copy its state and fill that set.
+// spread counters, and its collide blocks copy its state and file it there
+// for its frame 1.
 
 // Pointers loaded from a SceneDesc are generic; the round trip through the
 // global address space lets the compiler use global (not flat) accesses.
@@ -1575,6 +1590,7 @@ __device__ __forceinline__ RowBuckets scene_buckets(const SceneDesc &S)
     rb.spill = gptr(S.rb.spill);
     rb.fill = gptr(S.rb.fill);
     rb.spill_count = gptr(S.rb.spill_count);
+    rb.spread = gptr(S.rb.spread);
     rb.cap = S.rb.cap;
     return rb;
 }
@@ -1604,17 +1620,18 @@ __global__ __launch_bounds__(NT) void batch_row_sort_kernel(const SceneDesc *__r
     uint32_t *off = gptr(S.off);
     const RowBuckets rb = scene_buckets(S);
     if (time == 0) {
-        // INIT: cell 0 holds every sphere; clear bucket set 1 for the INIT step
+        // INIT: cell 0 holds every sphere; clear the spread counters and
+        // spill count 1 for the INIT step
         for (uint32_t c = threadIdx.x; c < G; c += NT)
             off[m.y * G + c] = (m.y == 0u && c == 0u) ? 0u : S.n;
         if (threadIdx.x == 0) {
             if (m.y == G - 1u) off[G * G] = S.n;
-            rb.fill[(G + m.y) * kLine] = 0u;
+            rb.spread[m.y * kSpread] = 0u;
             if (m.y == 0u) rb.spill_count[kLine] = 0u;
         }
         return;
     }
-    row_sort_body<NT>(rb, (uint32_t)time & 1u, gptr(S.scratch_pairs), gptr(S.scratch_idx), G, S.n,
+    row_sort_body<NT>(rb, (uint32_t)time & 1u, time == 1, gptr(S.scratch_pairs), gptr(S.scratch_idx), G, S.n,
                       gptr(S.perm), off, gptr(S.keys), m.y);
 }
 
@@ -1701,6 +1718,8 @@ hipError_t launch_rekey(const PhysConsts &c, const float4 *pos, uint32_t *key_ou
                        set, c.grids);
     hipLaunchKernelGGL(rekey_kernel, dim3((n + 255u) / 256u), dim3(256), 0, stream, c, pos,
                        key_out, rb, set, n);
+    hipLaunchKernelGGL(spread_pack_kernel, dim3((c.grids + 255u) / 256u), dim3(256), 0, stream, rb,
+                       set, c.grids);
     return hipGetLastError();
 }
 

@@ -91,7 +91,7 @@ SceneLayout scene_layout(uint64_t n, uint64_t g)
     l.cap = mb::bucket_cap((uint32_t)n, (uint32_t)g);
     l.bucket = take(g * l.cap * 8);
     l.off = take((g * g + 1) * 4);
-    l.counters = take(mb::bucket_counter_words((uint32_t)g) * 4);  // fill [2][G] | spill counts [2]
+    l.counters = take(mb::bucket_counter_words((uint32_t)g) * 4);  // spread | fill [2][G] | spill [2]
     l.bytes = at;
     return l;
 }
@@ -241,8 +241,8 @@ struct Request {
         d.off = reinterpret_cast<uint32_t *>(region + layout.off);
         d.rb.bucket = reinterpret_cast<uint64_t *>(region + layout.bucket);
         d.rb.spill = reinterpret_cast<uint64_t *>(region + layout.spill);
-        d.rb.fill = reinterpret_cast<uint32_t *>(region + layout.counters);
-        d.rb.spill_count = d.rb.fill + 2 * (size_t)g * mb::kLine;
+        mb::bucket_counters(reinterpret_cast<uint32_t *>(region + layout.counters), (uint32_t)g,
+                            d.rb);
         d.rb.cap = layout.cap;
         d.scratch_pairs = reinterpret_cast<uint64_t *>(region + layout.scratch_pairs);
         d.scratch_idx = reinterpret_cast<uint32_t *>(region + layout.scratch);

@@ -78,8 +78,9 @@ def test_scene_bytes_covers_every_array():
     per_sphere = 16 * 4 + 4 * 3 + 8 + 8
     cap = max(4 * -(-n // g), 512)
     cap = -(-cap // 64) * 64
-    # buckets, cell offsets, and one packed word per fill / spill counter
-    want = per_sphere * n + 8 * g * cap + 4 * (g * g + 1) + 4 * (2 * g + 2)
+    # buckets, cell offsets, one packed word per fill / spill counter, and
+    # the spread counters of filings in input order (one per 128-byte line)
+    want = per_sphere * n + 8 * g * cap + 4 * (g * g + 1) + 4 * (2 * g + 2) + 128 * g
     got = Server.scene_bytes(n, g)
     assert want <= got <= want + 12 * 256
 
