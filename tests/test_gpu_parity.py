@@ -264,6 +264,38 @@ def test_settings_change_and_disable(gpu, oracle):
         assert g.time() == t
 
 
+@pytest.mark.parametrize("split", [1, 4])
+def test_regrid_rekeys_like_a_fresh_upload(gpu, oracle, split):
+    """A new grid mid-run re-keys the current state (its INIT-order filing
+    counts in the spread set; after frame 0 the counts are copied into the
+    set the next frame sorts).  Every LOOP frame after that must equal the
+    oracle's LOOP frames on the same state uploaded fresh (INIT only copies
+    it and keys it), frame by frame."""
+    cfg = replace(CONFIGS["plain_65k"], number_of_agents=8192, grids=64)
+    ps = cfg.settings()
+    pos, vf = generate(cfg, seed=11)
+    with Physics(ps) as g:
+        g.upload_state(pos, vf)
+        g.run(split)
+        p_mid, v_mid = g.download_state()
+        # same map (G * grid_size = 64), coarser cells
+        ps2 = replace(ps, number_of_grids_one_dimension=32, grid_size=2)
+        g.set_settings(ps2)
+        r = oracle.World(ps2)
+        r.upload(p_mid, v_mid)
+        r.step(1)  # INIT
+        for f in range(4):
+            g.run(1)
+            r.step(1)
+            d, rd = g.debug(), r.debug()
+            assert np.array_equal(d["perm"], rd["perm"]), f"perm differs, frame {f}"
+            assert np.array_equal(d["next_keys"], rd["next_keys"]), f"keys differ, frame {f}"
+            pg, vg = g.download_state()
+            pr, vr = r.download()
+            assert_state_close(pg, vg, pr, vr)
+        r.close()
+
+
 def test_edge_cases_tiny_worlds(gpu, oracle):
     rng = np.random.default_rng(8)
     # one sphere, one cell
