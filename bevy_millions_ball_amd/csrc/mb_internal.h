@@ -115,12 +115,12 @@ constexpr uint32_t kStagePoolBig = 1536; // the same for dense scenes (fewer blo
 // the fills of the rows before its own) while it clears set !(t & 1), which
 // the collide of frame t then fills.  Pairs beyond a bucket's capacity spill
 // into one list (row << 42 | column << 30 | slot).
-// The counters are packed (kLine words apart, one word): the collide
+// The counters are packed (kFillStride = 1 word apart): the collide
 // kernel reserves with one returning atomic per row per block, so neighbouring
 // rows sharing a line costs it nothing measurable, while the row sort's
 // prefix over the counters of all lower rows reads 32 counters per line
 // (the frame at 1M spheres: 49.1 us one per 128-byte line, 46.9 us packed).
-constexpr uint32_t kLine = 1;
+constexpr uint32_t kFillStride = 1;
 // Filings of spheres in input order (the INIT step, a re-key) cannot be
 // aggregated per block: each wave files into ~64 rows with one atomic per
 // row.  They count in a third set, one counter per 128-byte line (kSpread
@@ -132,8 +132,8 @@ constexpr uint32_t kSpread = 32;
 struct RowBuckets {
     uint64_t *bucket;         // G x cap pairs (column << 32 | slot)
     uint64_t *spill;          // n entries
-    uint32_t *fill;           // 2 x G counters, set-major, kLine words apart
-    uint32_t *spill_count;    // 2 counters, kLine words apart
+    uint32_t *fill;           // 2 x G counters, set-major, kFillStride words apart
+    uint32_t *spill_count;    // 2 counters, kFillStride words apart
     uint32_t *spread;         // G counters, kSpread words apart: filings in input order
     uint32_t cap;             // pairs per bucket
 };
@@ -142,7 +142,7 @@ struct RowBuckets {
 // spill counts, in that order.
 inline size_t bucket_counter_words(uint32_t g)
 {
-    return (size_t)g * kSpread + (2 * (size_t)g + 2) * kLine;
+    return (size_t)g * kSpread + (2 * (size_t)g + 2) * kFillStride;
 }
 
 // Counter pointers of G rows laid out as bucket_counter_words describes.
@@ -150,7 +150,7 @@ inline void bucket_counters(uint32_t *counters, uint32_t g, RowBuckets &rb)
 {
     rb.spread = counters;
     rb.fill = counters + (size_t)g * kSpread;
-    rb.spill_count = rb.fill + 2 * (size_t)g * kLine;
+    rb.spill_count = rb.fill + 2 * (size_t)g * kFillStride;
 }
 
 // Bucket capacity for n spheres over G rows: four times the mean row, at

@@ -1003,7 +1003,7 @@ __device__ __forceinline__ void block_file(BlockFiling &f, const RowBuckets &rb,
         const uint32_t rank = atomicAdd(&f.count[w], 1u);
         entry = (w << 24) | (rank << 12) | col;
     } else {
-        bucket_file<kLine>(rb, fill, spill_count, row, col, s0 + tid);
+        bucket_file<kFillStride>(rb, fill, spill_count, row, col, s0 + tid);
     }
     f.entry[tid] = entry;
     __threadfence_block();
@@ -1019,7 +1019,7 @@ __device__ __forceinline__ void block_file(BlockFiling &f, const RowBuckets &rb,
     const uint32_t nact = (uint32_t)__popcll(act);
     for (uint32_t i = li; i < kFileWin; i += nact) {
         const uint32_t cnt = f.count[i];
-        if (cnt) f.base[i] = atomicAdd(&fill[(row0 + i - kFileLo) * kLine], cnt);
+        if (cnt) f.base[i] = atomicAdd(&fill[(row0 + i - kFileLo) * kFillStride], cnt);
     }
     __threadfence_block();
     for (uint32_t i = li; i <= last; i += nact) {
@@ -1068,16 +1068,16 @@ __device__ __forceinline__ void row_sort_body(
     __shared__ uint32_t s_scratch[NT / 64u];
     __shared__ uint32_t s_spilled;
     const uint32_t tid = threadIdx.x;
-    const uint32_t *fill_in = spread_in ? rb.spread : rb.fill + set * grids * kLine;
-    const uint32_t stride = spread_in ? kSpread : kLine;
+    const uint32_t *fill_in = spread_in ? rb.spread : rb.fill + set * grids * kFillStride;
+    const uint32_t stride = spread_in ? kSpread : kFillStride;
     uint32_t part = 0;
     for (uint32_t q = tid; q < r; q += NT) part += fill_in[q * stride];
     uint32_t base;
     (void)block_exclusive_scan<NT>(part, s_scratch, base);  // base = spheres in rows < r
     const uint32_t m = fill_in[r * stride];
     if (tid == 0) {
-        rb.fill[((set ^ 1u) * grids + r) * kLine] = 0u;
-        if (r == 0) rb.spill_count[(set ^ 1u) * kLine] = 0u;
+        rb.fill[((set ^ 1u) * grids + r) * kFillStride] = 0u;
+        if (r == 0) rb.spill_count[(set ^ 1u) * kFillStride] = 0u;
         s_spilled = 0u;
     }
     for (uint32_t c = tid; c < grids; c += NT) s_cell[c] = 0u;
@@ -1087,7 +1087,7 @@ __device__ __forceinline__ void row_sort_body(
         // of the scratch array
         for (uint32_t e = tid; e < rb.cap; e += NT) scratch_pairs[base + e] = src[e];
         __syncthreads();
-        const uint32_t spilled = rb.spill_count[set * kLine];
+        const uint32_t spilled = rb.spill_count[set * kFillStride];
         for (uint32_t i = tid; i < spilled; i += NT) {
             const uint64_t v = rb.spill[i];
             if ((uint32_t)(v >> 42) == r) {
@@ -1423,7 +1423,7 @@ __device__ __forceinline__ void physics_step_body(
     jitter(a, s, ox, oy, oz);
     const uint32_t next = finish_agent<SPRING>(c, a, s, pos_o, vfme_o, keys);
     const uint32_t nz = fast_div(next, c.grids_div);
-    block_file(s_file, rb, rb.fill + set_out * G * kLine, rb.spill_count + set_out * kLine,
+    block_file(s_file, rb, rb.fill + set_out * G * kFillStride, rb.spill_count + set_out * kFillStride,
                fast_div(kfirst, c.grids_div), nz, next - nz * G, s0, last);
 }
 
@@ -1458,7 +1458,7 @@ __device__ __forceinline__ void physics_init_body(
     vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
                            pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
     key_out[s] = cz * c.grids + cx;
-    bucket_file<kSpread>(rb, rb.spread, rb.spill_count + kLine, cz, cx, s);  // read by frame 1
+    bucket_file<kSpread>(rb, rb.spread, rb.spill_count + kFillStride, cz, cx, s);  // read by frame 1
 }
 
 __global__ __launch_bounds__(256) void physics_init_step_kernel(
@@ -1480,7 +1480,7 @@ __global__ __launch_bounds__(256) void init_offsets_kernel(uint32_t *__restrict_
     const uint32_t c = blockIdx.x * 256u + threadIdx.x;
     if (c <= cells) off[c] = c == 0 ? 0u : n;
     if (c < grids) rb.spread[c * kSpread] = 0u;
-    if (c == 0) rb.spill_count[kLine] = 0u;
+    if (c == 0) rb.spill_count[kFillStride] = 0u;
 }
 
 // A re-key: clears the spread counters and spill count `set` (first kernel),
@@ -1492,14 +1492,14 @@ __global__ __launch_bounds__(256) void bucket_clear_kernel(RowBuckets rb, uint32
 {
     const uint32_t r = blockIdx.x * 256u + threadIdx.x;
     if (r < grids) rb.spread[r * kSpread] = 0u;
-    if (r == 0) rb.spill_count[set * kLine] = 0u;
+    if (r == 0) rb.spill_count[set * kFillStride] = 0u;
 }
 
 __global__ __launch_bounds__(256) void spread_pack_kernel(RowBuckets rb, uint32_t set,
                                                           uint32_t grids)
 {
     const uint32_t r = blockIdx.x * 256u + threadIdx.x;
-    if (r < grids) rb.fill[(set * grids + r) * kLine] = rb.spread[r * kSpread];
+    if (r < grids) rb.fill[(set * grids + r) * kFillStride] = rb.spread[r * kSpread];
 }
 
 __global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *__restrict__ pos,
@@ -1512,7 +1512,7 @@ __global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *
     uint32_t cx, cz;
     cell_of(c, p.x, p.z, cx, cz);
     key_out[s] = cz * c.grids + cx;
-    bucket_file<kSpread>(rb, rb.spread, rb.spill_count + set * kLine, cz, cx, s);
+    bucket_file<kSpread>(rb, rb.spread, rb.spill_count + set * kFillStride, cz, cx, s);
 }
 
 template <uint32_t POOL>
@@ -1627,7 +1627,7 @@ __global__ __launch_bounds__(NT) void batch_row_sort_kernel(const SceneDesc *__r
         if (threadIdx.x == 0) {
             if (m.y == G - 1u) off[G * G] = S.n;
             rb.spread[m.y * kSpread] = 0u;
-            if (m.y == 0u) rb.spill_count[kLine] = 0u;
+            if (m.y == 0u) rb.spill_count[kFillStride] = 0u;
         }
         return;
     }
