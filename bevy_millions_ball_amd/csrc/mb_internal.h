@@ -182,8 +182,9 @@ struct FrameTuning {
 };
 
 // Enqueues frame `time`: INIT at time 0 (offsets, copy, keys, buckets for
-// frame 1), else LOOP (row sort of bucket set time & 1, collide + integrate
-// filing set !(time & 1)).  `events` (3 entries or null) are recorded before
+// frame 1 counted in the spread set), else LOOP (row sort of bucket set
+// time & 1, counted in the spread set at time 1, collide + integrate filing
+// set !(time & 1)).  `events` (3 entries or null) are recorded before
 // the row sort, before collide + integrate and after it.
 hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time, uint32_t n,
                         hipStream_t stream, hipEvent_t *events, FrameTuning tuning = {});

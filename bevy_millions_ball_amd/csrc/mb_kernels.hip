@@ -938,10 +938,11 @@ __device__ __forceinline__ uint32_t block_scan_lds(uint32_t *vals, uint32_t coun
     return total;
 }
 
-// Files (column, slot) under `row` in a bucket set: the lanes of a wave that
-// share a row take consecutive places with one returning atomic per distinct
-// row, the leaders of all rows issuing theirs at once.  A pair beyond the
-// bucket's capacity goes to the spill list.  Every active lane files one pair.
+// Files (column, slot) under `row` in a bucket set whose counters are STRIDE
+// words apart (fill[row * STRIDE]): the lanes of a wave that share a row take
+// consecutive places with one returning atomic per distinct row, the leaders
+// of all rows issuing theirs at once.  A pair beyond the bucket's capacity
+// goes to the spill list.  Every active lane files one pair.
 template <uint32_t STRIDE>
 __device__ __forceinline__ void bucket_file(const RowBuckets &rb, uint32_t *__restrict__ fill,
                                             uint32_t *__restrict__ spill_count, uint32_t row,
