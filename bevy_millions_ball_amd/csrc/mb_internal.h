@@ -125,7 +125,7 @@ constexpr uint32_t kFillStride = 1;
 // aggregated per block: each wave files into ~64 rows with one atomic per
 // row.  They count in a third set, one counter per 128-byte line (kSpread
 // words apart), where packed counters would queue 32 rows' atomics on a line
-// (INIT at 1M spheres: 207 us packed, 83 us spread).  The row sort of frame 1
+// (INIT at 1M spheres: 207 us packed, 74-79 us spread).  The row sort of frame 1
 // reads that set; a re-key copies it into the set its next frame reads.
 constexpr uint32_t kSpread = 32;
 

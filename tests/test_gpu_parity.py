@@ -213,6 +213,18 @@ def test_step_bit_exact_full_size(gpu, oracle, name, frames):
         compare_frames(g, r, frames)
 
 
+@pytest.mark.parametrize("name,frames", [("uniform_1m", 200), ("spring_clumps_1m", 60)])
+def test_long_run_bit_exact_full_size(gpu, oracle, name, frames):
+    """Hundreds of frames at full size, compared every 20 frames: the
+    trajectories are chaotic, so a single differing bit anywhere (a contact
+    order, a rounding, a filing) would spread and show."""
+    cfg = CONFIGS[name]
+    print(f"[long] {name} frames={frames}", flush=True)
+    g, r = make_pair(oracle, cfg, seed=7)
+    with g:
+        compare_frames(g, r, frames, check_every=20)
+
+
 def test_hundred_frames_statistics(gpu, oracle):
     """Over 100 frames the trajectories stay bit-identical; the kinetic energy
     and penetration statistics are then trivially equal, and are checked for
