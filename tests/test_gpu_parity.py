@@ -213,7 +213,8 @@ def test_step_bit_exact_full_size(gpu, oracle, name, frames):
         compare_frames(g, r, frames)
 
 
-@pytest.mark.parametrize("name,frames", [("uniform_1m", 200), ("spring_clumps_1m", 60)])
+@pytest.mark.parametrize("name,frames", [("uniform_1m", 200), ("spring_clumps_1m", 60),
+                                         ("uniform_4m", 40)])
 def test_long_run_bit_exact_full_size(gpu, oracle, name, frames):
     """Hundreds of frames at full size, compared every 20 frames: the
     trajectories are chaotic, so a single differing bit anywhere (a contact
