@@ -226,6 +226,19 @@ def test_long_run_bit_exact_full_size(gpu, oracle, name, frames):
         compare_frames(g, r, frames, check_every=20)
 
 
+def test_largest_grid_bit_exact(gpu, oracle):
+    """The largest grid the keys allow (G = 4096, G^2 = 2^24 cells) with
+    N = G^2 spheres: ~4096 spheres per row, so about half the rows exceed the
+    in-LDS row size and take the global-memory row sort."""
+    cfg = replace(CONFIGS["uniform_1m"], number_of_agents=1 << 24, grids=4096)
+    print("[max] G=4096 n=16777216", flush=True)
+    g, r = make_pair(oracle, cfg, seed=3)
+    with g:
+        compare_frames(g, r, 3)
+        rows = np.bincount(r.debug()["next_keys"] // 4096, minlength=4096)
+        assert rows.max() > 4096 > rows.min()
+
+
 def test_hundred_frames_statistics(gpu, oracle):
     """Over 100 frames the trajectories stay bit-identical; the kinetic energy
     and penetration statistics are then trivially equal, and are checked for
