@@ -1049,11 +1049,11 @@ __host__ __device__ constexpr uint32_t row_sort_cells(uint32_t grids, uint32_t n
 // One block per row r of frame `time` (>= 1).  The row's pairs are in bucket
 // set time & 1 (spilled pairs in the spill list), counted in that set or, for
 // frame 1 (`spread_in`), in the spread set the INIT step filled; its slots
-// start after the spheres of all lower rows.  Counting sort by column in LDS, then each index
-// is ranked among the indices of its cell, which makes the result exactly
-// the stable argsort of the keys.  Writes perm, the sorted keys and the
-// per-cell offsets of the row, and clears bucket set !(time & 1) for the
-// collide kernel that follows.  Dynamic LDS: `grids` cell counters (padded
+// start after the spheres of all lower rows.  Counting sort by column in LDS,
+// then each index is ranked among the indices of its cell, which makes the
+// result exactly the stable argsort of the keys.  Writes perm, the sorted
+// keys and the per-cell offsets of the row, and clears bucket set
+// !(time & 1) for the collide kernel that follows.  Dynamic LDS: `grids` cell counters (padded
 // to a multiple of 1024 words for the vector scan) followed by kRowCap
 // indices.
 template <uint32_t NT>
@@ -1175,8 +1175,8 @@ __global__ __launch_bounds__(kSortThreads) void row_sort_kernel(
     uint32_t *__restrict__ scratch_idx, uint32_t grids, uint32_t n,
     uint32_t *__restrict__ perm, uint32_t *__restrict__ off, uint32_t *__restrict__ sorted_keys)
 {
-    row_sort_body<kSortThreads>(rb, set, spread_in != 0u, scratch_pairs, scratch_idx, grids, n, perm, off,
-                                sorted_keys, xcd_block(blockIdx.x, gridDim.x));
+    row_sort_body<kSortThreads>(rb, set, spread_in != 0u, scratch_pairs, scratch_idx, grids, n,
+                                perm, off, sorted_keys, xcd_block(blockIdx.x, gridDim.x));
 }
 
 template <uint32_t NT = kSortThreads>
@@ -1424,8 +1424,9 @@ __device__ __forceinline__ void physics_step_body(
     jitter(a, s, ox, oy, oz);
     const uint32_t next = finish_agent<SPRING>(c, a, s, pos_o, vfme_o, keys);
     const uint32_t nz = fast_div(next, c.grids_div);
-    block_file(s_file, rb, rb.fill + set_out * G * kFillStride, rb.spill_count + set_out * kFillStride,
-               fast_div(kfirst, c.grids_div), nz, next - nz * G, s0, last);
+    block_file(s_file, rb, rb.fill + set_out * G * kFillStride,
+               rb.spill_count + set_out * kFillStride, fast_div(kfirst, c.grids_div), nz,
+               next - nz * G, s0, last);
 }
 
 template <bool SPRING, uint32_t POOL>
@@ -1632,8 +1633,8 @@ __global__ __launch_bounds__(NT) void batch_row_sort_kernel(const SceneDesc *__r
         }
         return;
     }
-    row_sort_body<NT>(rb, (uint32_t)time & 1u, time == 1, gptr(S.scratch_pairs), gptr(S.scratch_idx), G, S.n,
-                      gptr(S.perm), off, gptr(S.keys), m.y);
+    row_sort_body<NT>(rb, (uint32_t)time & 1u, time == 1, gptr(S.scratch_pairs),
+                      gptr(S.scratch_idx), G, S.n, gptr(S.perm), off, gptr(S.keys), m.y);
 }
 
 template <bool SPRING>
