@@ -105,6 +105,7 @@ mb::PhysConsts mb::compile_consts(const mb_settings &s)
     c.grid_size_div = mb::make_fastdiv(s.grid_size);
     c.grids_div = mb::make_fastdiv(s.number_of_grids_one_dimension);
     c.scan_margin = 1.0f;
+    c.gather_priority = s.max_number_of_agents > (1u << 21) ? 1u : 0u;
     return c;
 }
 

@@ -65,8 +65,14 @@ struct PhysConsts {
     int32_t enable_spring;
     FastDiv grid_size_div;   // u32 / GRID_SIZE
     FastDiv grids_div;       // cell key / NUMBER_OF_GRIDS_ONE_DIMENSION = row
-    // Scheduling choices that do not change results (autotuned):
-    float scan_margin;       // candidate-scan margin, as a fraction of the radius (> 0)
+    // Scheduling choices that do not change results:
+    float scan_margin;       // candidate-scan margin, as a fraction of the radius (> 0; autotuned)
+    // Launch the single-scene collide kernel that raises a block's wave
+    // priority until its windows are staged, so that its dependent gathers
+    // issue ahead of other blocks' walks.  Set for scenes above 2^21 spheres
+    // (at 4M: 151.9 -> 146.9 us per collide launch; at 1M, two and a third
+    // rounds of blocks, it gained nothing).
+    uint32_t gather_priority;
 };
 
 // ---- radix sort (8-bit digits, reduce-then-scan per pass) ----------------

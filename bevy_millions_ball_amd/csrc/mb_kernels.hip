@@ -1337,7 +1337,7 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
 // ranges (physics.wgsl:119-206) in the shader's order.  The block's first and
 // last cells come from the sorted keys, so the window bounds are uniform
 // loads issued at launch and one barrier separates staging from the walk.
-template <bool SPRING, uint32_t POOL>
+template <bool SPRING, uint32_t POOL, bool PRIO = false>
 __device__ __forceinline__ void physics_step_body(
     const PhysConsts &c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
     uint32_t *__restrict__ keys, const float4 *__restrict__ pos_i,
@@ -1346,6 +1346,7 @@ __device__ __forceinline__ void physics_step_body(
     float4 *__restrict__ s_pos, uint2 *__restrict__ s_vm)
 {
     __shared__ BlockFiling s_file;
+    if (PRIO) __builtin_amdgcn_s_setprio(2);
     const uint32_t tid = threadIdx.x;
     const uint32_t s0 = blk * 256u;
     const uint32_t last = min(n - s0, 256u) - 1u;
@@ -1411,6 +1412,7 @@ __device__ __forceinline__ void physics_step_body(
         }
     }
     __syncthreads();
+    if (PRIO) __builtin_amdgcn_s_setprio(0);
     if (!valid) return;
 
     const Ranges rg = make_ranges(lo0, hi0, lo1, hi1, lo2, hi2, wlo[0], wlo[1], wlo[2], wbase[1],
@@ -1429,7 +1431,7 @@ __device__ __forceinline__ void physics_step_body(
                next - nz * G, s0, last);
 }
 
-template <bool SPRING, uint32_t POOL>
+template <bool SPRING, uint32_t POOL, bool PRIO>
 __global__ __launch_bounds__(256) void physics_step_kernel(
     PhysConsts c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
     uint32_t *__restrict__ keys, const float4 *__restrict__ pos_i,
@@ -1438,7 +1440,8 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
 {
     __shared__ float4 s_pos[POOL];  // the three windows back to back
     __shared__ uint2 s_vm[POOL];    // their velocity and mass halves
-    physics_step_body<SPRING, POOL>(c, perm, off, keys, pos_i, vfme_i, pos_o, vfme_o, rb, set_out,
+    physics_step_body<SPRING, POOL, PRIO>(c, perm, off, keys, pos_i, vfme_i, pos_o, vfme_o, rb,
+                                          set_out,
                                     n, xcd_block(blockIdx.x, gridDim.x), s_pos, s_vm);
 }
 
@@ -1517,19 +1520,29 @@ __global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *
     bucket_file<kSpread>(rb, rb.spread, rb.spill_count + set * kFillStride, cz, cx, s);
 }
 
+template <uint32_t POOL, bool PRIO>
+static void launch_collide_prio(const FrameBuffers &b, const PhysConsts &c, uint32_t set_out,
+                                uint32_t n, hipStream_t stream)
+{
+    const dim3 grid((n + 255u) / 256u);
+    if (c.enable_spring)
+        hipLaunchKernelGGL((physics_step_kernel<true, POOL, PRIO>), grid, dim3(256), 0, stream, c,
+                           b.perm, b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, b.rb,
+                           set_out, n);
+    else
+        hipLaunchKernelGGL((physics_step_kernel<false, POOL, PRIO>), grid, dim3(256), 0, stream, c,
+                           b.perm, b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, b.rb,
+                           set_out, n);
+}
+
 template <uint32_t POOL>
 static void launch_collide(const FrameBuffers &b, const PhysConsts &c, uint32_t set_out,
                            uint32_t n, hipStream_t stream)
 {
-    const dim3 grid((n + 255u) / 256u);
-    if (c.enable_spring)
-        hipLaunchKernelGGL((physics_step_kernel<true, POOL>), grid, dim3(256), 0, stream, c, b.perm,
-                           b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, b.rb, set_out,
-                           n);
+    if (c.gather_priority)
+        launch_collide_prio<POOL, true>(b, c, set_out, n, stream);
     else
-        hipLaunchKernelGGL((physics_step_kernel<false, POOL>), grid, dim3(256), 0, stream, c,
-                           b.perm, b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, b.rb,
-                           set_out, n);
+        launch_collide_prio<POOL, false>(b, c, set_out, n, stream);
 }
 
 hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time, uint32_t n,
