@@ -455,6 +455,22 @@ def test_tuning_does_not_change_results(gpu, scene):
         assert np.array_equal(k2, runs[0][2]), t
 
 
+@pytest.mark.parametrize("pool", [896, 1536])
+def test_large_scene_collide_variant_bit_exact(gpu, oracle, pool):
+    """Scenes above 2^21 spheres launch the collide variant that raises a
+    block's wave priority while it stages its windows; its spring and plain
+    instantiations, with either pool, must give the oracle's frames."""
+    n = (1 << 21) + 4096
+    for name in ("spring_clumps_1m", "uniform_1m"):
+        cfg = replace(CONFIGS[name], number_of_agents=n, grids=1448)
+        print(f"[large] {name} n={n} pool={pool}", flush=True)
+        g, r = make_pair(oracle, cfg, seed=13)
+        with g:
+            g.set_tuning(1.0, pool)
+            compare_frames(g, r, 3)
+        r.close()
+
+
 def test_autotune_restores_state_and_keeps_results(gpu):
     cfg = replace(CONFIGS["uniform_1m"], number_of_agents=20_000, grids=128)
     ps = cfg.settings()
