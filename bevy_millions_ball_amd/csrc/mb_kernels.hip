@@ -1056,7 +1056,7 @@ __host__ __device__ constexpr uint32_t row_sort_cells(uint32_t grids, uint32_t n
 // !(time & 1) for the collide kernel that follows.  Dynamic LDS: `grids` cell counters (padded
 // to a multiple of 1024 words for the vector scan) followed by kRowCap
 // indices.
-template <uint32_t NT>
+template <uint32_t NT, uint32_t CAP = kRowCap>
 __device__ __forceinline__ void row_sort_body(
     const RowBuckets &rb, uint32_t set, bool spread_in, uint64_t *__restrict__ scratch_pairs,
     uint32_t *__restrict__ scratch_idx, uint32_t grids, uint32_t n,
@@ -1101,8 +1101,8 @@ __device__ __forceinline__ void row_sort_body(
     }
     __syncthreads();
 
-    if (m <= kRowCap) {
-        constexpr uint32_t kPer = kRowCap / NT;
+    if (m <= CAP) {
+        constexpr uint32_t kPer = CAP / NT;
         uint64_t el[kPer];
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) {
@@ -1179,10 +1179,10 @@ __global__ __launch_bounds__(kSortThreads) void row_sort_kernel(
                                 perm, off, sorted_keys, xcd_block(blockIdx.x, gridDim.x));
 }
 
-template <uint32_t NT = kSortThreads>
+template <uint32_t NT = kSortThreads, uint32_t CAP = kRowCap>
 static size_t row_sort_lds(uint32_t grids)
 {
-    return ((size_t)row_sort_cells(grids, NT) + kRowCap) * 4u;
+    return ((size_t)row_sort_cells(grids, NT) + CAP) * 4u;
 }
 
 // solve_sphere_constraint behind a cheap conservative filter:
@@ -1621,7 +1621,7 @@ __device__ __forceinline__ uint32_t plan_frame(uint32_t k, const uint32_t *count
     return counter ? *counter : k;
 }
 
-template <uint32_t NT>
+template <uint32_t NT, uint32_t CAP>
 __global__ __launch_bounds__(NT) void batch_row_sort_kernel(const SceneDesc *__restrict__ scenes,
                                                             const uint2 *__restrict__ map,
                                                             uint32_t k,
@@ -1646,7 +1646,7 @@ __global__ __launch_bounds__(NT) void batch_row_sort_kernel(const SceneDesc *__r
         }
         return;
     }
-    row_sort_body<NT>(rb, (uint32_t)time & 1u, time == 1, gptr(S.scratch_pairs),
+    row_sort_body<NT, CAP>(rb, (uint32_t)time & 1u, time == 1, gptr(S.scratch_pairs),
                       gptr(S.scratch_idx), G, S.n, gptr(S.perm), off, gptr(S.keys), m.y);
 }
 
@@ -1703,16 +1703,19 @@ hipError_t launch_batch_frame(const BatchPlan &p, uint32_t k, bool from_counter,
     const uint32_t *counter = from_counter ? p.frame_counter : nullptr;
     if (p.scene_count == 0) return hipSuccess;
     if (p.rows) {
-        // rows of at most 1024 spheres on average: 256-thread blocks, more
-        // of them resident at once (each row costs about the same latency);
-        // denser rows: kSortThreads, more threads per row
+        // rows of at most 1024 spheres on average: 256-thread blocks with
+        // room for 2048 spheres of a row in LDS (fewer registers and less
+        // LDS, so more rows resident at once; each row costs about the same
+        // latency, and a longer row is sorted in global memory); denser
+        // rows: kSortThreads and kRowCap
         if (p.max_row_mean <= 1024u)
-            hipLaunchKernelGGL(batch_row_sort_kernel<256>, dim3(p.rows), dim3(256),
-                               row_sort_lds<256>(p.max_grids), stream, p.scenes, p.row_map, k,
-                               counter);
+            hipLaunchKernelGGL((batch_row_sort_kernel<256, kRowCapNarrow>), dim3(p.rows), dim3(256),
+                               (row_sort_lds<256, kRowCapNarrow>(p.max_grids)), stream, p.scenes,
+                               p.row_map, k, counter);
         else
-            hipLaunchKernelGGL(batch_row_sort_kernel<kSortThreads>, dim3(p.rows), dim3(kSortThreads),
-                               row_sort_lds(p.max_grids), stream, p.scenes, p.row_map, k, counter);
+            hipLaunchKernelGGL((batch_row_sort_kernel<kSortThreads, kRowCap>), dim3(p.rows),
+                               dim3(kSortThreads), row_sort_lds(p.max_grids), stream, p.scenes,
+                               p.row_map, k, counter);
     }
     if (p.blk_plain_count)
         hipLaunchKernelGGL(batch_physics_step_kernel<false>, dim3(p.blk_plain_count), dim3(256), 0,
