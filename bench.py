@@ -223,6 +223,24 @@ def serving_trace(n_requests: int, seed: int) -> int:
     return 0
 
 
+def copy_calibration(launches: int = 5) -> int:
+    """A plain device copy of known size (64 MiB read + 64 MiB written per
+    launch), run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE: known bytes
+    divided by the counter's KiB scale the kernels' counters to bytes."""
+    import torch
+
+    copy_bytes = 64 * 1024 * 1024
+    torch.cuda.set_device(0)
+    x = torch.empty(copy_bytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(launches):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"launches": launches, "bytes_read": copy_bytes,
+                                           "bytes_written": copy_bytes}}), flush=True)
+    return 0
+
+
 def other_config(name: str, seed: int, steps: int) -> dict:
     """Throughput and per-kernel device time of another workload (the
     parity-test configurations), `steps` LOOP frames after INIT + 10 and the
@@ -277,6 +295,8 @@ def main() -> int:
     ap.add_argument("--no-other-configs", action="store_true")
     ap.add_argument("--serving-trace", action="store_true",
                     help="only one batched drain of the serving mix (for a kernel trace)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only a plain 64 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -290,6 +310,8 @@ def main() -> int:
 
     if args.serving_trace:
         return serving_trace(args.serving_requests, args.seed)
+    if args.copy_calibration:
+        return copy_calibration()
 
     cfg = CONFIGS[args.config]
     n = cfg.number_of_agents

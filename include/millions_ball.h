@@ -57,7 +57,23 @@ typedef struct mb_settings {
  * bevy_radix_sort EVE/ODD global keys/vals handles (mod.rs:27-32).  Each
  * position_with_scale buffer holds N float4 (xyz, w = radius); each
  * packed_vfme buffer holds N uint4 of pack2x16float pairs
- * (vx|vy, vz|fx, fy|fz, mass|0), physics.wgsl:235-242. */
+ * (vx|vy, vz|fx, fy|fz, mass|0), physics.wgsl:235-242.
+ *
+ * Roles of the sort buffers after frame t, with p = t & 1 and q = 1 - p:
+ *   global_keys[q]  the cell keys the frame wrote for the next sort, one per
+ *                   sphere of state q, in slot order (as in the reference);
+ *   global_vals[q]  the frame's permutation: sorted slot -> index into the
+ *                   frame's input state (as in the reference);
+ *   global_keys[p], global_vals[p]  NOT start / close.  The reference reuses
+ *                   them for the per-cell ranges (mod.rs:659-705); the frame
+ *                   path here keeps the ranges as per-cell offsets in a buffer
+ *                   of its own, so after such a frame global_keys[p] still
+ *                   holds the frame's input keys and global_vals[p] the
+ *                   previous frame's permutation.  Only frames run through
+ *                   the reference-order stage kernels (an INIT state with
+ *                   cells outside the grid and the frame after it, the frame
+ *                   after a grid change) leave start / close there.  Read the
+ *                   ranges with mb_physics_debug_download. */
 typedef struct mb_buffers {
     void *position_with_scale[2];
     void *packed_vfme[2];
