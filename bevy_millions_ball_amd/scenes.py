@@ -124,11 +124,139 @@ def generate(cfg: SceneConfig, seed: int = 0) -> Tuple[np.ndarray, np.ndarray]:
     return pos, pack_vfme(vel, force, mass)
 
 
+# ---- the demo's seeded velocity stream -------------------------------------
+# mod.rs:199-212 draws each sphere's velocity with
+# `ChaCha8Rng::seed_from_u64(0)` and `rng.gen_range(-0.8..0.8)` twice (x, then
+# z).  Those crates are not in the reference tree; their published algorithms
+# at the versions Cargo.lock pins are restated here:
+#   rand_core 0.6  SeedableRng::seed_from_u64: PCG32 (XSH-RR; multiplier
+#                  6364136223846793005, increment 11634580027462260723; the
+#                  state is advanced before each output) fills the 32-byte
+#                  seed four little-endian bytes at a time;
+#   rand_chacha 0.3.1  ChaCha8Rng: the ChaCha block function with 8 rounds,
+#                  key = seed, a 64-bit block counter from 0 in words 12-13 and
+#                  stream 0 in words 14-15; next_u32 reads the keystream as
+#                  little-endian words, block after block;
+#   rand 0.8.5     UniformFloat<f32>::sample_single(low, high): draw a u32,
+#                  keep its top 23 bits as the mantissa of a float in [1, 2),
+#                  subtract 1, then res = value * (high - low) + low (two f32
+#                  roundings, no fused multiply-add); redraw while res >= high.
+
+_CHACHA_CONST = np.array([0x61707865, 0x3320646E, 0x79622D32, 0x6B206574], np.uint32)
+_PCG_MUL = 6364136223846793005
+_PCG_INC = 11634580027462260723
+_M64 = (1 << 64) - 1
+
+
+def pcg_xsh_rr(state: int) -> int:
+    """PCG's XSH-RR output function: 32 bits from a 64-bit state."""
+    xorshifted = (((state >> 18) ^ state) >> 27) & 0xFFFFFFFF
+    rot = state >> 59
+    return ((xorshifted >> rot) | (xorshifted << ((32 - rot) & 31))) & 0xFFFFFFFF
+
+
+def pcg32_seed_bytes(state: int, length: int = 32) -> bytes:
+    """rand_core 0.6 SeedableRng::seed_from_u64's seed bytes."""
+    out = bytearray()
+    state &= _M64
+    while len(out) < length:
+        state = (state * _PCG_MUL + _PCG_INC) & _M64
+        out += pcg_xsh_rr(state).to_bytes(4, "little")
+    return bytes(out[:length])
+
+
+def _rotl(v, c):
+    return (v << np.uint32(c)) | (v >> np.uint32(32 - c))
+
+
+def chacha_blocks(key: bytes, first_block: int, count: int, rounds: int = 8,
+                  stream: int = 0) -> np.ndarray:
+    """`count` consecutive ChaCha blocks (16 little-endian u32 words each,
+    block counter first_block, first_block + 1, ...), flattened."""
+    kw = np.frombuffer(key, dtype="<u4").astype(np.uint32)
+    ctr = np.arange(first_block, first_block + count, dtype=np.uint64)
+    init = np.empty((16, count), np.uint32)
+    init[0:4] = _CHACHA_CONST[:, None]
+    init[4:12] = kw[:, None]
+    init[12] = (ctr & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    init[13] = (ctr >> np.uint64(32)).astype(np.uint32)
+    init[14] = np.uint32(stream & 0xFFFFFFFF)
+    init[15] = np.uint32((stream >> 32) & 0xFFFFFFFF)
+    x = [init[i].copy() for i in range(16)]
+
+    def qr(a, b, c, d):
+        x[a] += x[b]; x[d] ^= x[a]; x[d] = _rotl(x[d], 16)
+        x[c] += x[d]; x[b] ^= x[c]; x[b] = _rotl(x[b], 12)
+        x[a] += x[b]; x[d] ^= x[a]; x[d] = _rotl(x[d], 8)
+        x[c] += x[d]; x[b] ^= x[c]; x[b] = _rotl(x[b], 7)
+
+    for _ in range(rounds // 2):
+        qr(0, 4, 8, 12); qr(1, 5, 9, 13); qr(2, 6, 10, 14); qr(3, 7, 11, 15)
+        qr(0, 5, 10, 15); qr(1, 6, 11, 12); qr(2, 7, 8, 13); qr(3, 4, 9, 14)
+    out = np.stack([x[i] + init[i] for i in range(16)])  # (16, count)
+    return out.T.reshape(-1)
+
+
+class ChaCha8U32:
+    """rand_chacha 0.3.1 ChaCha8Rng::seed_from_u64(seed) as a u32 stream."""
+
+    def __init__(self, seed: int, rounds: int = 8):
+        self.key = pcg32_seed_bytes(seed)
+        self.rounds = rounds
+        self.block = 0
+        self.buf = np.empty(0, np.uint32)
+        self.pos = 0
+
+    def take(self, count: int) -> np.ndarray:
+        while self.buf.shape[0] - self.pos < count:
+            need = count - (self.buf.shape[0] - self.pos)
+            blocks = max(4, (need + 15) // 16)
+            more = chacha_blocks(self.key, self.block, blocks, self.rounds)
+            self.block += blocks
+            self.buf = np.concatenate([self.buf[self.pos:], more])
+            self.pos = 0
+        out = self.buf[self.pos:self.pos + count]
+        self.pos += count
+        return out
+
+
+def gen_range_f32(words: np.ndarray, low: float, high: float) -> np.ndarray:
+    """rand 0.8.5 UniformFloat<f32>::sample_single on successive u32 draws,
+    without the redraw (candidates; the caller drops those >= high)."""
+    lo = np.float32(low)
+    hi = np.float32(high)
+    scale = np.float32(hi - lo)
+    value1_2 = ((words >> np.uint32(9)) | np.uint32(127 << 23)).view(np.float32)
+    value0_1 = value1_2 - np.float32(1.0)
+    return (value0_1 * scale).astype(np.float32) + lo
+
+
+def demo_velocities(n: int, seed: int = 0) -> np.ndarray:
+    """Velocities of packed_vfme_data (mod.rs:199-212): x and z drawn in that
+    order per sphere from ChaCha8 seeded `seed`, U[-0.8, 0.8); y = 0."""
+    rng = ChaCha8U32(seed)
+    need = 2 * n
+    out = np.empty(need, np.float32)
+    got = 0
+    while got < need:
+        cand = gen_range_f32(rng.take(need - got), -0.8, 0.8)
+        ok = cand < np.float32(0.8)
+        good = cand.shape[0] if ok.all() else int(np.argmin(ok))
+        out[got:got + good] = cand[:good]
+        got += good
+        if good < cand.shape[0]:
+            # the rejected draw is consumed; the words after it are drawn again
+            rng.pos -= cand.shape[0] - good - 1
+    vel = np.zeros((n, 3), np.float32)
+    vel[:, 0] = out[0::2]
+    vel[:, 2] = out[1::2]
+    return vel
+
+
 def reference_lattice(max_number_of_agents: int, seed: int = 0):
     """The demo's start state (mod.rs:172-212): an XZ lattice with spacing 4 at
-    y = 4, radius 1, velocities U[-0.8, 0.8) in x and z.  The reference draws
-    the velocities with ChaCha8 seeded 0; numpy's PCG64 is used here, so the
-    lattice is exact and the velocities are only distributed alike."""
+    y = 4, radius 1, velocities U[-0.8, 0.8) in x and z from ChaCha8 seeded 0
+    (demo_velocities), mass 1, force 0."""
     n = max_number_of_agents
     log = int(n).bit_length() - 1
     log_x = log // 2
@@ -142,8 +270,5 @@ def reference_lattice(max_number_of_agents: int, seed: int = 0):
     pos[:, 1] = 4.0
     pos[:, 2] = gz.reshape(-1) * np.float32(4.0)
     pos[:, 3] = 1.0
-    rng = np.random.default_rng(seed)
-    vel = np.zeros((n, 3), np.float32)
-    vel[:, 0] = rng.uniform(-0.8, 0.8, n)
-    vel[:, 2] = rng.uniform(-0.8, 0.8, n)
+    vel = demo_velocities(n, seed)
     return pos[:n], pack_vfme(vel, np.zeros((n, 3), np.float32), np.ones(n, np.float32))

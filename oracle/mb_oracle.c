@@ -242,16 +242,18 @@ void mbo_hash3f1u(uint32_t n, float out[3])
     out[2] = (float)(v * 0x3FFFFu) / inv;
 }
 
-/* get_collision_grid_id / _index, physics.wgsl:107-117 (plus the clamp). */
+/* get_collision_grid_id, physics.wgsl:107-109: vec2u(pos.xz + HALF_MAP_SIZE)
+ * / GRID_SIZE, not clamped.  A position beyond the map's +x / +z edge gives a
+ * coordinate >= G, and its key (below) aliases a cell of a later row, as in
+ * the shader; one beyond the -x / -z edge saturates to 0. */
 static inline void cell_id(const mbo_consts *c, float x, float z, uint32_t *cx,
                            uint32_t *cz)
 {
-    uint32_t ix = f32_to_u32_sat(x + c->half_map_size) / c->grid_size;
-    uint32_t iz = f32_to_u32_sat(z + c->half_map_size) / c->grid_size;
-    *cx = ix < c->grids - 1u ? ix : c->grids - 1u;
-    *cz = iz < c->grids - 1u ? iz : c->grids - 1u;
+    *cx = f32_to_u32_sat(x + c->half_map_size) / c->grid_size;
+    *cz = f32_to_u32_sat(z + c->half_map_size) / c->grid_size;
 }
 
+/* get_collision_grid_index, physics.wgsl:111-113 (u32 arithmetic, wrapping). */
 uint32_t mbo_cell_key(const mbo_consts *c, float x, float z)
 {
     uint32_t cx, cz;
@@ -317,9 +319,29 @@ typedef struct {
     const uint32_t *perm;
     const uint32_t *start;
     const uint32_t *close;
+    uint32_t cells;      /* entries of start / close */
+    uint32_t n;          /* entries of perm, pos_i, vfme_i */
     const float *pos_i;
     const uint32_t *vfme_i;
+    uint32_t *trace;     /* when set: sorted slots visited, in order */
+    uint32_t trace_cap;
+    uint32_t trace_len;
 } loop_ctx;
+
+/* Reads of start / close.  The reference's arrays are N long and it never
+ * indexes past them for a state whose cells all lie in the grid; a cell key
+ * from a position far outside the map can (WGSL then returns an
+ * implementation-defined value).  Here such a read is an empty cell (start
+ * INVALID, close 0); the library rejects every state that would reach it. */
+static inline uint32_t rd_start(const loop_ctx *x, uint32_t i)
+{
+    return i < x->cells ? x->start[i] : MBO_INVALID;
+}
+
+static inline uint32_t rd_close(const loop_ctx *x, uint32_t i)
+{
+    return i < x->cells ? x->close[i] : 0u;
+}
 
 /* solve_sphere_constraint, physics.wgsl:300-331. */
 static inline void solve_sphere_constraint(const mbo_consts *c, agent *a, const agent *b)
@@ -355,9 +377,15 @@ static inline void collide_range(const loop_ctx *x, agent *a, uint32_t first,
                                  uint32_t last, uint32_t skip)
 {
     /* `for (var i = first; i <= last; i++)`; 64-bit counter so that
-     * last == 0xFFFFFFFF cannot wrap. */
-    for (uint64_t i = first; i <= (uint64_t)last; ++i) {
+     * last == 0xFFFFFFFF cannot wrap.  Slots past N (out of the reference's
+     * bounds, see rd_start) end the range. */
+    for (uint64_t i = first; i <= (uint64_t)last && i < x->n; ++i) {
         if ((uint32_t)i == skip) continue;
+        if (x->trace) {
+            loop_ctx *m = (loop_ctx *)x;
+            if (m->trace_len < m->trace_cap) m->trace[m->trace_len] = (uint32_t)i;
+            m->trace_len++;
+        }
         agent b = agent_new(x->pos_i, x->vfme_i, x->perm[i]);
         solve_sphere_constraint(x->c, a, &b);
     }
@@ -371,21 +399,21 @@ static inline void handle_collision(const loop_ctx *x, agent *a, uint32_t cx,
     uint32_t first = MBO_INVALID;
     uint32_t last = 0u;
     if (cx > 0u) {
-        uint32_t st = x->start[base - 1u];
-        if (st != MBO_INVALID) { first = st; last = x->close[base - 1u]; }
+        uint32_t st = rd_start(x, base - 1u);
+        if (st != MBO_INVALID) { first = st; last = rd_close(x, base - 1u); }
     }
     {
-        uint32_t st = x->start[base];
+        uint32_t st = rd_start(x, base);
         if (st != MBO_INVALID) {
             if (first == MBO_INVALID) first = st;
-            last = x->close[base];
+            last = rd_close(x, base);
         }
     }
     if (cx < G - 1u) {
-        uint32_t st = x->start[base + 1u];
+        uint32_t st = rd_start(x, base + 1u);
         if (st != MBO_INVALID) {
             if (first == MBO_INVALID) first = st;
-            last = x->close[base + 1u];
+            last = rd_close(x, base + 1u);
         }
     }
     collide_range(x, a, first, last, MBO_INVALID);
@@ -398,15 +426,15 @@ static inline void handle_collisions(const loop_ctx *x, agent *a, uint32_t s,
     const uint32_t G = x->c->grids;
     v3 old_position = a->position;
     {
-        uint32_t first = x->start[k];
-        uint32_t last = x->close[k];
+        uint32_t first = rd_start(x, k);
+        uint32_t last = rd_close(x, k);
         if (cx > 0u) {
-            uint32_t st = x->start[k - 1u];
+            uint32_t st = rd_start(x, k - 1u);
             if (st != MBO_INVALID) first = st;
         }
         if (cx < G - 1u) {
-            uint32_t st = x->start[k + 1u];
-            if (st != MBO_INVALID) last = x->close[k + 1u];
+            uint32_t st = rd_start(x, k + 1u);
+            if (st != MBO_INVALID) last = rd_close(x, k + 1u);
         }
         collide_range(x, a, first, last, s);
     }
@@ -502,16 +530,33 @@ static void physics_slot(const loop_ctx *x, int init, uint32_t s, uint32_t *key_
 }
 
 void mbo_physics_run(const mbo_consts *c, int init, const uint32_t *perm,
-                     const uint32_t *start, const uint32_t *close,
+                     const uint32_t *start, const uint32_t *close, uint32_t cells,
                      uint32_t *key_out, const float *pos_i,
                      const uint32_t *vfme_i, float *pos_o, uint32_t *vfme_o,
                      uint32_t n, int threads)
 {
-    loop_ctx x = {c, perm, start, close, pos_i, vfme_i};
+    loop_ctx x = {c, perm, start, close, cells, n, pos_i, vfme_i, NULL, 0u, 0u};
     (void)threads;
 #pragma omp parallel for num_threads(threads > 0 ? threads : 1) schedule(dynamic, 4096)
     for (int64_t s = 0; s < (int64_t)n; ++s)
         physics_slot(&x, init, (uint32_t)s, key_out, pos_o, vfme_o);
+}
+
+uint32_t mbo_physics_candidates(const mbo_consts *c, const uint32_t *perm,
+                                const uint32_t *start, const uint32_t *close,
+                                uint32_t cells, const float *pos_i,
+                                const uint32_t *vfme_i, uint32_t n, uint32_t s,
+                                uint32_t *out, uint32_t cap)
+{
+    loop_ctx x = {c, perm, start, close, cells, n, pos_i, vfme_i, out, cap, 0u};
+    uint32_t *key_out = (uint32_t *)calloc((size_t)n, sizeof(uint32_t));
+    float *pos_o = (float *)calloc((size_t)n * 4u, sizeof(float));
+    uint32_t *vfme_o = (uint32_t *)calloc((size_t)n * 4u, sizeof(uint32_t));
+    physics_slot(&x, 0, s, key_out, pos_o, vfme_o);
+    free(key_out);
+    free(pos_o);
+    free(vfme_o);
+    return x.trace_len;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -587,9 +632,12 @@ void mbo_spatial_hashing(const uint32_t *keys, uint32_t *start, uint32_t *close,
         uint32_t k = keys[i];
         uint32_t prv = 0xFFFFFFFFu - k;
         if (i > 0u) prv = keys[i - 1u];
-        if (k != prv) start[k] = i;
         uint32_t nxt = 0xFFFFFFFFu - k;
         if (i < n - 1u) nxt = keys[i + 1u];
+        /* a key past the arrays would be written out of bounds by the
+         * shader; it is skipped (the library rejects such states) */
+        if (k >= zero_count) continue;
+        if (k != prv) start[k] = i;
         if (k != nxt) close[k] = i;
     }
 }
@@ -710,8 +758,8 @@ void mbo_world_step(mbo_world *w, uint32_t steps, int threads, double *stage_sec
         double t2 = now_seconds();
         /* physics::run: INIT at time 0, else LOOP (mod.rs:929-939) */
         mbo_physics_run(&w->c, w->time == 0, w->vals[q], w->keys[p], w->vals[p],
-                        w->keys[q], w->pos[p], w->vfme[p], w->pos[q], w->vfme[q],
-                        w->n, threads);
+                        (uint32_t)w->cells, w->keys[q], w->pos[p], w->vfme[p], w->pos[q],
+                        w->vfme[q], w->n, threads);
         double t3 = now_seconds();
         if (stage_seconds) {
             stage_seconds[0] += t1 - t0;

@@ -16,8 +16,13 @@
  *     operand, ties yield the second operand), sign(0) = +0;
  *   - f32 -> f16 is round-to-nearest-even, f16 -> f32 exact;
  *   - u32(f32) truncates toward zero and saturates, NaN -> 0;
- *   - the cell coordinate is clamped to [0, G-1] (the reference indexes out of
- *     bounds there; in-box positions never reach the clamp).
+ *   - the cell coordinate is NOT clamped (physics.wgsl:107-117): a position
+ *     past the map's +x / +z edge has a coordinate >= G and its key aliases a
+ *     cell of a later row, exactly as in the shader;
+ *   - a read of start / close past the arrays (only keys of positions far
+ *     outside the map reach one; the shader's result is adapter-defined there)
+ *     is an empty cell, and a key past the arrays is not written by spatial
+ *     hashing.  The library rejects every state that would reach either.
  * The physics pass is pinned by closed forms and invariants; at the level of
  * a wgpu adapter's own rounding of physics.wgsl it is parity unpinned (no
  * adapter run is available to compare with).
@@ -92,12 +97,22 @@ void mbo_spatial_hashing(const uint32_t *sorted_keys, uint32_t *start,
                          uint32_t *close, uint32_t n, uint32_t zero_count,
                          int threads);
 
-/* One dispatch of physics.wgsl (INIT_PIPELINE when init != 0, else LOOP). */
+/* One dispatch of physics.wgsl (INIT_PIPELINE when init != 0, else LOOP);
+ * start / close hold `cells` entries. */
 void mbo_physics_run(const mbo_consts *c, int init, const uint32_t *perm,
-                     const uint32_t *start, const uint32_t *close,
+                     const uint32_t *start, const uint32_t *close, uint32_t cells,
                      uint32_t *key_out, const float *pos_i,
                      const uint32_t *vfme_i, float *pos_o, uint32_t *vfme_o,
                      uint32_t n, int threads);
+
+/* The sorted slots whose spheres the LOOP pass of slot s tests against, in
+ * the order it tests them (handle_collisions, physics.wgsl:119-206): the
+ * first `cap` go to `out`; returns how many there are. */
+uint32_t mbo_physics_candidates(const mbo_consts *c, const uint32_t *perm,
+                                const uint32_t *start, const uint32_t *close,
+                                uint32_t cells, const float *pos_i,
+                                const uint32_t *vfme_i, uint32_t n, uint32_t s,
+                                uint32_t *out, uint32_t cap);
 
 /* A whole world with the reference's ping-pong buffers and time counter. */
 typedef struct mbo_world mbo_world;

@@ -7,7 +7,14 @@ Pinned by (tests/test_oracle_*.py):
   spatial hashing  - the same known answers (start/close closed forms);
   quantisation     - closed-form f32 values of the shader constants;
   physics          - closed forms (free fall, wall bounce, head-on restitution,
-                     overlap push, spring force, hash values) and invariants.
+                     overlap push, spring force, hash values) and invariants;
+                     the neighbour walk by closed-form pairs across each of the
+                     eight neighbour directions and the grid's edges, and by the
+                     visited candidate list against a brute-force restatement of
+                     the 3 x 3 neighbourhood (tests/test_oracle_walk.py), which
+                     mutated copies of the walk fail (tests/test_oracle_mutants.py);
+  cell key         - the reference's unclamped key, on the demo lattice of
+                     main.rs:42-49 and on points outside the map.
 Physics at the level of the adapter's own rounding is parity unpinned: only a
 wgpu run of physics.wgsl could fix it, and none is available.
 """
@@ -22,6 +29,9 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libmb_oracle.so"
+# Another build of the same source may be named here (the mutation tests load
+# deliberately broken copies of mb_oracle.c to show that the walk checks fail).
+LIB_OVERRIDE = os.environ.get("MB_ORACLE_LIB")
 
 
 class OSettings(C.Structure):
@@ -73,9 +83,12 @@ def build() -> Path:
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
-        if not LIB_PATH.exists():
-            build()
-        L = C.CDLL(str(LIB_PATH))
+        if LIB_OVERRIDE:
+            L = C.CDLL(LIB_OVERRIDE)
+        else:
+            if not LIB_PATH.exists():
+                build()
+            L = C.CDLL(str(LIB_PATH))
         L.mbo_settings_default.argtypes = [C.POINTER(OSettings)]
         L.mbo_compile_consts.argtypes = [C.POINTER(OSettings), C.POINTER(OConsts)]
         L.mbo_f32_to_f16.argtypes = [C.c_float]
@@ -89,8 +102,11 @@ def lib() -> C.CDLL:
         L.mbo_cell_key.restype = C.c_uint32
         L.mbo_radix_sort.argtypes = [_P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]
         L.mbo_spatial_hashing.argtypes = [_P, _P, _P, C.c_uint32, C.c_uint32, C.c_int]
-        L.mbo_physics_run.argtypes = [C.POINTER(OConsts), C.c_int, _P, _P, _P, _P, _P, _P, _P,
-                                      _P, C.c_uint32, C.c_int]
+        L.mbo_physics_run.argtypes = [C.POINTER(OConsts), C.c_int, _P, _P, _P, C.c_uint32, _P,
+                                      _P, _P, _P, _P, C.c_uint32, C.c_int]
+        L.mbo_physics_candidates.argtypes = [C.POINTER(OConsts), _P, _P, _P, C.c_uint32, _P, _P,
+                                             C.c_uint32, C.c_uint32, _P, C.c_uint32]
+        L.mbo_physics_candidates.restype = C.c_uint32
         L.mbo_world_create.argtypes = [C.POINTER(OSettings)]
         L.mbo_world_create.restype = _P
         L.mbo_world_destroy.argtypes = [_P]
@@ -193,12 +209,28 @@ def physics_run(ps, init, perm, start, close, pos_i, vfme_i, threads=None):
     key_out = np.empty(n, np.uint32)
     pos_o = np.empty((n, 4), np.float32)
     vfme_o = np.empty((n, 4), np.uint32)
+    cells = 0 if start is None else int(min(len(start), len(close)))
     lib().mbo_physics_run(C.byref(c), int(bool(init)), _p(np.ascontiguousarray(perm, np.uint32)),
-                          _p(start), _p(close), _p(key_out),
+                          _p(start), _p(close), cells, _p(key_out),
                           _p(np.ascontiguousarray(pos_i, np.float32)),
                           _p(np.ascontiguousarray(vfme_i, np.uint32)), _p(pos_o), _p(vfme_o), n,
                           threads or default_threads())
     return key_out, pos_o, vfme_o
+
+
+def physics_candidates(ps, perm, start, close, pos_i, vfme_i, slot, cap=4096):
+    """Sorted slots the LOOP pass of `slot` tests against, in its order."""
+    c = compile_consts(ps)
+    perm = np.ascontiguousarray(perm, np.uint32)
+    start = np.ascontiguousarray(start, np.uint32)
+    close = np.ascontiguousarray(close, np.uint32)
+    pos_i = np.ascontiguousarray(pos_i, np.float32)
+    vfme_i = np.ascontiguousarray(vfme_i, np.uint32)
+    out = np.zeros(cap, np.uint32)
+    cnt = lib().mbo_physics_candidates(C.byref(c), _p(perm), _p(start), _p(close),
+                                       int(min(len(start), len(close))), _p(pos_i), _p(vfme_i),
+                                       perm.shape[0], int(slot), _p(out), cap)
+    return out[:min(int(cnt), cap)].copy()
 
 
 class World:

@@ -69,9 +69,13 @@ def test_cell_key_bruteforce(oracle):
     assert oracle.cell_key(c, -127.0, -128.0) == 1
     assert oracle.cell_key(c, -128.0, -127.0) == 256
     assert oracle.cell_key(c, 127.75, 127.75) == 256 * 256 - 1
-    # outside the grid the cell is clamped (the reference indexes out of bounds)
-    assert oracle.cell_key(c, 500.0, 0.0) == 128 * 256 + 255
+    # outside the grid the coordinate is not clamped (physics.wgsl:107-117):
+    # x = 500 -> u32(628) = column 628 of row 128, aliasing a cell of row 130
+    assert oracle.cell_key(c, 500.0, 0.0) == 128 * 256 + 628 == 33396
+    assert oracle.cell_key(c, 0.0, 200.0) == 328 * 256 + 128
+    # below the map u32(f32) saturates at 0; NaN converts to 0
     assert oracle.cell_key(c, -500.0, float("nan")) == 0
+    assert oracle.cell_key(c, -500.0, 0.0) == 128 * 256
 
 
 def test_hash3f1u_closed_forms(oracle):
@@ -228,8 +232,8 @@ def test_frame_invariants(oracle, cfg_name):
             order = np.argsort(prev_keys, kind="stable").astype(np.uint32)
             assert np.array_equal(d["perm"], order)
         nk = d["next_keys"]
-        cx = np.minimum(((p[:, 0] + np.float32(hms)).astype(np.float32)).astype(np.int64), cfg.grids - 1)
-        cz = np.minimum(((p[:, 2] + np.float32(hms)).astype(np.float32)).astype(np.int64), cfg.grids - 1)
+        cx = ((p[:, 0] + np.float32(hms)).astype(np.float32)).astype(np.int64)
+        cz = ((p[:, 2] + np.float32(hms)).astype(np.float32)).astype(np.int64)
         assert np.array_equal(nk.astype(np.int64), cz * cfg.grids + cx)
         if frame > 0 and not cfg.enable_spring:
             r = p[:, 3]
