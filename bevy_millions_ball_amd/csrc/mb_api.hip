@@ -5,6 +5,7 @@
 // per frame (mod.rs:659-705), the `time` counter (mod.rs:711-743, 804-839) and
 // the three stage entry points (bevy_radix_sort::run, spatial_hashing::run,
 // physics::run).  Kernels live in mb_kernels.hip.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -109,6 +110,58 @@ mb::PhysConsts mb::compile_consts(const mb_settings &s)
     return c;
 }
 
+// u32(x + HALF_MAP_SIZE) / GRID_SIZE in f32 then saturating u32 arithmetic,
+// as get_collision_grid_id (physics.wgsl:107-109) forms it; u64 out.
+uint64_t mb::host_cell(const PhysConsts &c, float x)
+{
+    const float s = x + c.half_map_size;
+    const uint64_t u = !(s > 0.0f) ? 0u : (s >= 4294967296.0f ? 0xFFFFFFFFull : (uint64_t)s);
+    return u / c.grid_size;
+}
+
+// solve_cube_constraint (physics.wgsl:269-298) leaves every LOOP output with
+// |x|, |z| <= HALF_MAP_SIZE - r (rigid walls: a sphere is clamped there, or
+// was inside already), or <= HALF_MAP_SIZE - r + r * R (spring walls clamp
+// only past a penetration of r * R; the bound is widened by 2^-20 of itself
+// for the rounding of that comparison).  The bound is linear in r, so its
+// largest value over radii in [r_min, r_max] is at an end.  Returns the
+// largest cell coordinate that allows.
+uint64_t mb::loop_cell_bound(const PhysConsts &c, float r_min, float r_max)
+{
+    uint64_t best = 0;
+    for (float r : {r_min, r_max}) {
+        const float ex = c.half_map_size - r;
+        float lim;
+        if (!c.enable_spring) {
+            lim = std::fabs(ex);
+        } else {
+            const float mm = -(r * c.r_agent);
+            lim = std::fabs(ex - mm);
+            lim = lim + lim * 0x1p-20f + 0x1p-100f;
+        }
+        best = std::max(best, host_cell(c, lim));
+    }
+    return best;
+}
+
+// Largest start / close index the walk of physics.wgsl:119-160 reads for a
+// sphere with both cell coordinates in [0, cmax]: k + 1 (k when cx >= G - 1),
+// or k + G + 1 (k + G) when cz < G - 1.  Each branch grows with cx and cz, so
+// the largest value lies at cx, cz in {G - 2, G - 1, cmax}.
+uint64_t mb::walk_read_bound(uint64_t g, uint64_t cmax)
+{
+    uint64_t best = 0;
+    const uint64_t cand[3] = {g >= 2 ? g - 2 : 0, g - 1, cmax};
+    for (uint64_t cx : cand)
+        for (uint64_t cz : cand) {
+            if (cx > cmax || cz > cmax) continue;
+            const uint64_t k = cz * g + cx;
+            const uint64_t right = cx + 1 < g ? 1u : 0u;
+            best = std::max(best, cz + 1 < g ? k + g + right : k + right);
+        }
+    return best;
+}
+
 namespace {
 
 // The constraints the reference documents for its buffers and sort
@@ -170,6 +223,28 @@ struct mb_physics {
     mb::FrameTuning tuning;      // launch choices that do not change results
     float scan_margin = 1.0f;
 
+    // Which frames run on the frame path and which on the stage kernels
+    // (the reference's own order: LSD sort, spatial hashing, physics pass on
+    // start / close, in the reference's buffer roles).  The frame path covers
+    // states whose cells all lie in the grid; the stage kernels run
+    //   - every frame while a LOOP output can leave the grid (stage_always:
+    //     a radius or spring setting that lets a wall-clamped position reach
+    //     past the map's edge),
+    //   - frames with time <= stage_until: the INIT frame of a state with a
+    //     sphere outside the grid and the LOOP frame that sorts its keys, or
+    //     the frame after a grid change, which sorts the keys of the old grid
+    //     (mod.rs:808-837 recompiles the pipelines and goes on with the
+    //     buffers as they are).
+    bool valid = false;          // a state has been uploaded and accepted
+    bool stage_always = false;
+    int64_t stage_until = -1;
+    int64_t filed_for = -1;      // the frame whose row buckets are filed
+    bool last_stage = false;     // the last frame ran on the stage kernels
+    float r_min = 0.0f, r_max = 0.0f;  // radii of the state (frames copy them)
+    void *sort_ws = nullptr;     // stage-kernel sort workspace (allocated on first need)
+    mb::StateCheck *check = nullptr;   // device result of launch_state_check
+    hipStream_t last_stream = nullptr; // stream of the last stream-ordered call
+
     mb::RowBuckets buckets() const
     {
         mb::RowBuckets rb;
@@ -221,6 +296,8 @@ static void release(mb_physics *p)
     if (p->spill) (void)hipFree(p->spill);
     if (p->scratch_pairs) (void)hipFree(p->scratch_pairs);
     if (p->scratch_idx) (void)hipFree(p->scratch_idx);
+    if (p->sort_ws) (void)hipFree(p->sort_ws);
+    if (p->check) (void)hipFree(p->check);
     for (auto &fe : p->prof_events)
         for (hipEvent_t e : fe)
             if (e) (void)hipEventDestroy(e);
@@ -261,6 +338,83 @@ static int drain_profile(mb_physics *p)
     return MB_OK;
 }
 
+// Waits for the stream of the caller's last stream-ordered call, so that work
+// these synchronous entry points enqueue (on that stream) starts after it.
+static int sync_last(mb_physics *p)
+{
+    MB_HIP(hipStreamSynchronize(p->last_stream));
+    return MB_OK;
+}
+
+static int ensure_sort_ws(mb_physics *p)
+{
+    if (p->sort_ws) return MB_OK;
+    const size_t bytes = mb::sort_workspace_layout(p->n).bytes;
+    if (hipMalloc(&p->sort_ws, bytes) != hipSuccess)
+        return fail(MB_ERR_OUT_OF_MEMORY, "device allocation failed (sort workspace)");
+    MB_HIP(hipMemset(p->sort_ws, 0, bytes));
+    return MB_OK;
+}
+
+struct StateInfo {
+    uint64_t max_read = 0, max_key = 0;
+    float r_min = 0.0f, r_max = 0.0f;
+    uint32_t bad = 0;
+};
+
+static float float_from_order_key(uint32_t k)
+{
+    const uint32_t b = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+
+// Runs launch_state_check on a state (writing its cell keys to key_out when
+// given) and waits for the result.
+static int check_state(mb_physics *p, const mb::PhysConsts &c, const float4 *pos,
+                       const uint4 *vfme, uint32_t *key_out, hipStream_t st, StateInfo &info)
+{
+    const mb::StateCheck init{0ull, 0ull, 0xFFFFFFFFu, 0u, 0u, 0u};
+    MB_HIP(hipMemcpyAsync(p->check, &init, sizeof(init), hipMemcpyHostToDevice, st));
+    hipError_t e = mb::launch_state_check(c, pos, vfme, key_out, p->n, p->check, st);
+    if (e != hipSuccess) return fail_hip(e, "state check");
+    mb::StateCheck r;
+    MB_HIP(hipMemcpyAsync(&r, p->check, sizeof(r), hipMemcpyDeviceToHost, st));
+    MB_HIP(hipStreamSynchronize(st));
+    info.max_read = r.max_read;
+    info.max_key = r.max_key;
+    info.r_min = float_from_order_key(r.r_min_key);
+    info.r_max = float_from_order_key(r.r_max_key);
+    info.bad = r.bad;
+    return MB_OK;
+}
+
+// Accepts or rejects a state for settings `c` and N spheres.  The reference
+// reads start / close out of bounds (its arrays hold N entries) for a sphere
+// whose walk reaches index N; a state is accepted only when no frame can do
+// that.  *always tells whether every frame must run on the stage kernels
+// (LOOP outputs can leave the grid).
+static int accept_state(const mb::PhysConsts &c, uint32_t n, const StateInfo &info, bool *always)
+{
+    if (info.bad)
+        return fail(MB_ERR_INVALID_ARGUMENT, std::to_string(info.bad) +
+                                                 " spheres with a non-finite position, radius or f16 half");
+    if (info.max_read >= n)
+        return fail(MB_ERR_INVALID_ARGUMENT,
+                    "a sphere lies so far past the map that its neighbour walk reads cell " +
+                        std::to_string(info.max_read) + " of " + std::to_string(n) +
+                        " (the reference reads start / close out of bounds there)");
+    const uint64_t cmax = mb::loop_cell_bound(c, info.r_min, info.r_max);
+    *always = cmax >= c.grids;
+    if (*always && mb::walk_read_bound(c.grids, cmax) >= n)
+        return fail(MB_ERR_INVALID_ARGUMENT,
+                    "the radii let a wall-clamped position reach cell coordinate " +
+                        std::to_string(cmax) + ", whose neighbour walk reads past the " +
+                        std::to_string(n) + " cells of start / close");
+    return MB_OK;
+}
+
 int mb_physics_create(const mb_settings *settings, mb_physics **out)
 {
     if (!settings || !out) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
@@ -290,6 +444,7 @@ int mb_physics_create(const mb_settings *settings, mb_physics **out)
     ok = ok && hipMalloc(&p->spill, n * 8u) == hipSuccess;
     ok = ok && hipMalloc(&p->scratch_pairs, n * 8u) == hipSuccess;
     ok = ok && hipMalloc(&p->scratch_idx, n * 4u) == hipSuccess;
+    ok = ok && hipMalloc(&p->check, sizeof(mb::StateCheck)) == hipSuccess;
     if (!ok) {
         release(p);
         delete p;
@@ -341,22 +496,39 @@ int mb_physics_set_settings(mb_physics *p, const mb_settings *s)
     if (rc != MB_OK) return rc;
     const bool regrid = s->number_of_grids_one_dimension != p->settings.number_of_grids_one_dimension ||
                         s->grid_size != p->settings.grid_size;
-    p->settings = *s;
-    p->consts = mb::compile_consts(*s);
-    p->consts.scan_margin = p->scan_margin;
-    if (regrid && p->time >= 0) {
-        // The next frame sorts keys of the new grid (the reference would sort
-        // the stale keys of the old grid once, reading stale `close` words).
-        DeviceGuard g(p->device);
+    mb::PhysConsts c = mb::compile_consts(*s);
+    c.scan_margin = p->scan_margin;
+    DeviceGuard g(p->device);
+    rc = sync_last(p);
+    if (rc != MB_OK) return rc;
+    bool always = false;
+    int64_t until = p->stage_until;
+    if (p->valid) {
+        // The frame after the change reads the current state: its positions
+        // give the cells of the new grid, and its keys (written by the last
+        // frame under the old grid) are what that frame sorts.
         const int cur = (int)((p->time + 1) & 1);
-        int rc2 = alloc_buckets(p);
-        if (rc2 != MB_OK) return rc2;
-        // the next frame (time + 1) sorts bucket set (time + 1) & 1
-        hipError_t e = mb::launch_rekey(p->consts, p->pos[cur], p->keys[cur], p->buckets(),
-                                        (uint32_t)((p->time + 1) & 1), p->n, nullptr);
-        if (e != hipSuccess) return fail_hip(e, "rekey");
-        MB_HIP(hipStreamSynchronize(nullptr));
+        StateInfo info;
+        rc = check_state(p, c, p->pos[cur], p->vfme[cur], nullptr, p->last_stream, info);
+        if (rc != MB_OK) return rc;
+        rc = accept_state(c, p->n, info, &always);
+        if (rc != MB_OK) return rc;  // settings unchanged
+        if (p->time < 0) {
+            // not started: the INIT frame keys the state in the new grid
+            until = info.max_key >= (uint64_t)c.grids * c.grids ? 1 : -1;
+        } else if (regrid) {
+            // the next frame sorts the old grid's keys, as the reference does
+            until = p->time + 1;
+        }
     }
+    p->settings = *s;
+    p->consts = c;
+    p->stage_always = always;
+    p->stage_until = until;
+    if (regrid) p->filed_for = -1;
+    rc = alloc_buckets(p);
+    if (rc != MB_OK) return rc;
+    if (p->stage_always || p->stage_until > p->time) return ensure_sort_ws(p);
     return MB_OK;
 }
 
@@ -371,6 +543,7 @@ int mb_physics_upload_state(mb_physics *p, const float *pos, const uint32_t *vfm
     }
     DeviceGuard g(p->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    p->valid = false;
     MB_HIP(hipMemcpyAsync(p->pos[0], pos, (size_t)p->n * sizeof(float4), hipMemcpyHostToDevice, st));
     MB_HIP(hipMemcpyAsync(p->vfme[0], vfme, (size_t)p->n * sizeof(uint4), hipMemcpyHostToDevice, st));
     return mb_physics_upload_state_device(p, nullptr, nullptr, stream);
@@ -381,7 +554,10 @@ int mb_physics_upload_state_device(mb_physics *p, const void *pos, const void *v
     if (!p) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
     DeviceGuard g(p->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    p->last_stream = st;
     const size_t n = p->n;
+    p->valid = false;
+    p->time = -1;
     if (pos) MB_HIP(hipMemcpyAsync(p->pos[0], pos, n * sizeof(float4), hipMemcpyDeviceToDevice, st));
     if (vfme) MB_HIP(hipMemcpyAsync(p->vfme[0], vfme, n * sizeof(uint4), hipMemcpyDeviceToDevice, st));
     MB_HIP(hipMemsetAsync(p->pos[1], 0, n * sizeof(float4), st));
@@ -391,8 +567,23 @@ int mb_physics_upload_state_device(mb_physics *p, const void *pos, const void *v
         MB_HIP(hipMemsetAsync(p->vals[i], 0, n * 4u, st));
     }
     MB_HIP(hipMemsetAsync(p->counters, 0, kCounterWords * 4u, st));
-    MB_HIP(hipStreamSynchronize(st));
-    p->time = -1;
+    StateInfo info;
+    int rc = check_state(p, p->consts, p->pos[0], p->vfme[0], nullptr, st, info);
+    if (rc != MB_OK) return rc;
+    bool always = false;
+    rc = accept_state(p->consts, p->n, info, &always);
+    if (rc != MB_OK) return rc;  // the handle holds no state until an upload is accepted
+    p->r_min = info.r_min;
+    p->r_max = info.r_max;
+    p->stage_always = always;
+    // INIT keys a sphere outside the grid: INIT and frame 1 (which sorts those
+    // keys) run on the stage kernels; frame 1's output lies in the grid
+    const uint64_t cells = (uint64_t)p->consts.grids * p->consts.grids;
+    p->stage_until = info.max_key >= cells ? 1 : -1;
+    p->filed_for = -1;
+    p->last_stage = false;
+    p->valid = true;
+    if (p->stage_always || p->stage_until >= 0) return ensure_sort_ws(p);
     return MB_OK;
 }
 
@@ -401,6 +592,7 @@ int mb_physics_download_state(const mb_physics *p, float *pos, uint32_t *vfme, v
     if (!p) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
     DeviceGuard g(p->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    const_cast<mb_physics *>(p)->last_stream = st;
     const int cur = (int)((p->time + 1) & 1);
     if (pos) MB_HIP(hipMemcpyAsync(pos, p->pos[cur], (size_t)p->n * sizeof(float4), hipMemcpyDeviceToHost, st));
     if (vfme) MB_HIP(hipMemcpyAsync(vfme, p->vfme[cur], (size_t)p->n * sizeof(uint4), hipMemcpyDeviceToHost, st));
@@ -445,10 +637,34 @@ int parse_checkpoint(const void *buffer, size_t size, mb_settings *settings, int
 
 }  // namespace
 
+// One frame in the reference's order and buffer roles (mod.rs:846-912):
+// bevy_radix_sort::run(.., N, 0..3, true, is_even) sorts keys[par] into
+// keys[q] / vals[q]; spatial_hashing::run writes start into keys[par] and
+// close into vals[par] (clearing N starts); physics::run (INIT at time 0,
+// else LOOP) reads them and writes state[q] and the next keys into keys[q].
+static hipError_t launch_stage_frame(mb_physics *p, int64_t t, hipStream_t st, hipEvent_t *ev)
+{
+    const int par = (int)(t & 1);
+    const int q = 1 - par;
+    const uint32_t n = p->n;
+    auto mark = [&](int i) { return ev ? hipEventRecord(ev[i], st) : hipSuccess; };
+    hipError_t e = mark(0);
+    if (e == hipSuccess)
+        e = mb::launch_radix_sort(p->sort_ws, n, p->keys[par], nullptr, p->keys[q], p->vals[q], n,
+                                  0u, 3u, st);
+    if (e == hipSuccess) e = mb::launch_spatial_hashing(p->keys[q], p->keys[par], p->vals[par], n, n, st);
+    if (e == hipSuccess) e = mark(1);
+    if (e == hipSuccess)
+        e = mb::launch_physics(p->consts, t == 0, p->vals[q], p->keys[par], p->vals[par], n,
+                               p->keys[q], p->pos[par], p->vfme[par], p->pos[q], p->vfme[q], n, st);
+    if (e == hipSuccess) e = mark(2);
+    return e;
+}
+
 static int run_frame(mb_physics *p, hipStream_t st)
 {
-    p->time += 1;
-    const int par = (int)(p->time & 1);
+    const int64_t t = p->time + 1;
+    const int par = (int)(t & 1);
     const int q = 1 - par;
     std::array<hipEvent_t, 3> *ev = nullptr;
     if (p->profiling) {
@@ -464,6 +680,23 @@ static int run_frame(mb_physics *p, hipStream_t st)
         }
         ev = &p->prof_events[p->prof_used++];
     }
+    if (p->stage_always || t <= p->stage_until) {
+        int rc = ensure_sort_ws(p);
+        if (rc != MB_OK) return rc;
+        hipError_t e = launch_stage_frame(p, t, st, ev ? ev->data() : nullptr);
+        if (e != hipSuccess) return fail_hip(e, "stage frame");
+        p->time = t;
+        p->last_stage = true;
+        return MB_OK;
+    }
+    if (t >= 1 && p->filed_for != t) {
+        // this frame's input was not filed by a frame-path frame (it is the
+        // output of a stage frame, a loaded checkpoint or a grid change):
+        // file it now, keying it exactly as that frame did
+        hipError_t e = mb::launch_rekey(p->consts, p->pos[par], p->keys[par], p->buckets(),
+                                        (uint32_t)par, p->n, st);
+        if (e != hipSuccess) return fail_hip(e, "rekey");
+    }
     mb::FrameBuffers b;
     b.keys_out = p->keys[q];
     b.perm = p->vals[q];
@@ -475,9 +708,12 @@ static int run_frame(mb_physics *p, hipStream_t st)
     b.vfme_i = p->vfme[par];
     b.pos_o = p->pos[q];
     b.vfme_o = p->vfme[q];
-    hipError_t e = mb::launch_frame(b, p->consts, p->time, p->n, st,
-                                    ev ? ev->data() : nullptr, p->tuning);
+    hipError_t e = mb::launch_frame(b, p->consts, t, p->n, st, ev ? ev->data() : nullptr,
+                                    p->tuning);
     if (e != hipSuccess) return fail_hip(e, "frame");
+    p->time = t;
+    p->filed_for = t + 1;
+    p->last_stage = false;
     return MB_OK;
 }
 
@@ -485,8 +721,10 @@ int mb_physics_step(mb_physics *p, uint32_t frames, void *stream)
 {
     if (!p) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
     if (!p->settings.enable_simulation) return MB_OK;
+    if (!p->valid) return fail(MB_ERR_STATE, "no accepted state: upload one first");
     DeviceGuard g(p->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    p->last_stream = st;
     for (uint32_t f = 0; f < frames; ++f) {
         int rc = run_frame(p, st);
         if (rc != MB_OK) return rc;
@@ -515,7 +753,8 @@ int mb_physics_save_checkpoint(const mb_physics *p, void *buffer, size_t size)
     std::memcpy(b + 16 + sizeof(mb_settings), &p->n, 4);
     float *pos = reinterpret_cast<float *>(b + kCkHeader);
     uint32_t *vfme = reinterpret_cast<uint32_t *>(b + kCkHeader + (size_t)p->n * 16u);
-    return mb_physics_download_state(p, pos, vfme, nullptr);
+    // on the stream of the last frame, so the copy sees that frame's output
+    return mb_physics_download_state(p, pos, vfme, p->last_stream);
 }
 
 int mb_physics_load_checkpoint(mb_physics *p, const void *buffer, size_t size)
@@ -534,27 +773,38 @@ int mb_physics_load_checkpoint(mb_physics *p, const void *buffer, size_t size)
     const char *b = static_cast<const char *>(buffer);
     const float *pos = reinterpret_cast<const float *>(b + kCkHeader);
     const uint32_t *vfme = reinterpret_cast<const uint32_t *>(b + kCkHeader + (size_t)n * 16u);
+    DeviceGuard g(p->device);
+    rc = sync_last(p);
+    if (rc != MB_OK) return rc;
+    hipStream_t st = p->last_stream;
     // settings first (buckets sized for the grid), then the state as uploaded
     p->settings = s;
     p->consts = mb::compile_consts(s);
     p->consts.scan_margin = p->scan_margin;
     rc = alloc_buckets(p);
     if (rc != MB_OK) return rc;
-    rc = mb_physics_upload_state(p, pos, vfme, nullptr);  // state in EVE, time = -1
+    rc = mb_physics_upload_state(p, pos, vfme, st);  // state in EVE, time = -1
     if (rc != MB_OK) return rc;
     if (t < 0) return MB_OK;
-    DeviceGuard g(p->device);
-    // frame t wrote state[(t + 1) & 1]; put the state there and file it for frame t + 1
+    // frame t wrote state[(t + 1) & 1] and, in keys[(t + 1) & 1], the cell
+    // keys frame t + 1 sorts: put the state there and key it
     const int cur = (int)((t + 1) & 1);
     if (cur != 0) {
-        MB_HIP(hipMemcpy(p->pos[cur], p->pos[0], (size_t)n * sizeof(float4), hipMemcpyDeviceToDevice));
-        MB_HIP(hipMemcpy(p->vfme[cur], p->vfme[0], (size_t)n * sizeof(uint4), hipMemcpyDeviceToDevice));
+        MB_HIP(hipMemcpyAsync(p->pos[cur], p->pos[0], (size_t)n * sizeof(float4),
+                              hipMemcpyDeviceToDevice, st));
+        MB_HIP(hipMemcpyAsync(p->vfme[cur], p->vfme[0], (size_t)n * sizeof(uint4),
+                              hipMemcpyDeviceToDevice, st));
     }
-    hipError_t e = mb::launch_rekey(p->consts, p->pos[cur], p->keys[cur], p->buckets(),
-                                    (uint32_t)((t + 1) & 1), n, nullptr);
-    if (e != hipSuccess) return fail_hip(e, "checkpoint rekey");
-    MB_HIP(hipStreamSynchronize(nullptr));
+    StateInfo info;
+    rc = check_state(p, p->consts, p->pos[cur], p->vfme[cur], p->keys[cur], st, info);
+    if (rc != MB_OK) return rc;
+    const uint64_t cells = (uint64_t)p->consts.grids * p->consts.grids;
+    // a state outside the grid (a checkpoint of an INIT frame) is sorted by
+    // the stage kernels; the frame path files the state lazily otherwise
+    p->stage_until = info.max_key >= cells ? t + 1 : -1;
+    p->filed_for = -1;
     p->time = t;
+    if (p->stage_always || p->stage_until > t) return ensure_sort_ws(p);
     return MB_OK;
 }
 
@@ -583,7 +833,10 @@ int mb_physics_autotune(mb_physics *p, uint32_t frames, mb_tuning *best, double 
 {
     if (!p) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
     if (frames == 0) return fail(MB_ERR_INVALID_ARGUMENT, "frames_per_trial must be >= 1");
+    if (!p->valid) return fail(MB_ERR_STATE, "no accepted state: upload one first");
     DeviceGuard g(p->device);
+    int rc0 = sync_last(p);
+    if (rc0 != MB_OK) return rc0;
     std::vector<char> ck(mb_checkpoint_size(p->n));
     int rc = mb_physics_save_checkpoint(p, ck.data(), ck.size());
     if (rc != MB_OK) return rc;
@@ -670,17 +923,26 @@ int mb_physics_debug_download(const mb_physics *p, uint32_t *perm, uint32_t *sta
     const size_t cells = (size_t)p->settings.number_of_grids_one_dimension *
                          p->settings.number_of_grids_one_dimension;
     std::vector<uint32_t> off;
-    if (start || close) {
+    if ((start || close) && !p->last_stage) {
         off.resize(cells + 1);
         MB_HIP(hipMemcpyAsync(off.data(), p->off, (cells + 1) * 4u, hipMemcpyDeviceToHost, st));
+    }
+    if (p->last_stage) {
+        // the stage kernels left start / close where the reference does
+        if (start) MB_HIP(hipMemcpyAsync(start, p->keys[par], n * 4u, hipMemcpyDeviceToHost, st));
+        if (close) MB_HIP(hipMemcpyAsync(close, p->vals[par], n * 4u, hipMemcpyDeviceToHost, st));
     }
     if (perm) MB_HIP(hipMemcpyAsync(perm, p->vals[q], n * 4u, hipMemcpyDeviceToHost, st));
     if (next_keys) MB_HIP(hipMemcpyAsync(next_keys, p->keys[q], n * 4u, hipMemcpyDeviceToHost, st));
     MB_HIP(hipStreamSynchronize(st));
     // start/close as spatial_hashing.wgsl leaves them: start is INVALID for
-    // empty cells (and for the entries past the last cell); close is defined
+    // empty cells (and for the entries past the last cell); close is reported
     // only where start is valid (0 elsewhere).
     for (size_t c = 0; (start || close) && c < n; ++c) {
+        if (p->last_stage) {
+            if (close && start && start[c] == MB_INVALID_INDEX) close[c] = 0u;
+            continue;
+        }
         const bool in = c < cells && off[c + 1] > off[c];
         if (start) start[c] = in ? off[c] : MB_INVALID_INDEX;
         if (close) close[c] = in ? off[c + 1] - 1u : 0u;
@@ -755,10 +1017,12 @@ int mb_physics_run(const mb_settings *s, int init, const uint32_t *perm, const u
         return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
     if (s->number_of_grids_one_dimension == 0 || s->grid_size == 0)
         return fail(MB_ERR_INVALID_ARGUMENT, "grid count and grid size must be >= 1");
-    hipError_t e = mb::launch_physics(mb::compile_consts(*s), init != 0, perm, start, close, key_out,
-                                      static_cast<const float4 *>(pos_i), static_cast<const uint4 *>(vfme_i),
-                                      static_cast<float4 *>(pos_o), static_cast<uint4 *>(vfme_o), n,
-                                      static_cast<hipStream_t>(stream));
+    const uint64_t g2 = (uint64_t)s->number_of_grids_one_dimension * s->number_of_grids_one_dimension;
+    const uint32_t cells = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(g2, n), 0xFFFFFFFFull);
+    hipError_t e = mb::launch_physics(mb::compile_consts(*s), init != 0, perm, start, close, cells,
+                                      key_out, static_cast<const float4 *>(pos_i),
+                                      static_cast<const uint4 *>(vfme_i), static_cast<float4 *>(pos_o),
+                                      static_cast<uint4 *>(vfme_o), n, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return fail_hip(e, "physics");
     return MB_OK;
 }

@@ -102,10 +102,27 @@ hipError_t launch_spatial_hashing(const uint32_t *sorted_keys, uint32_t *start,
                                   uint32_t *close, uint32_t n, uint32_t zero_count,
                                   hipStream_t stream);
 
+// start / close hold `cells` entries (reads past them are empty cells).
 hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
-                          const uint32_t *start, const uint32_t *close, uint32_t *key_out,
-                          const float4 *pos_i, const uint4 *vfme_i, float4 *pos_o,
-                          uint4 *vfme_o, uint32_t n, hipStream_t stream);
+                          const uint32_t *start, const uint32_t *close, uint32_t cells,
+                          uint32_t *key_out, const float4 *pos_i, const uint4 *vfme_i,
+                          float4 *pos_o, uint4 *vfme_o, uint32_t n, hipStream_t stream);
+
+// Result of launch_state_check; the host sets max_* and bad to 0, r_min_key
+// to 0xFFFFFFFF and r_max_key to 0 before the launch.  Radii are kept as
+// order-preserving u32 keys (sign bit flipped for positives, all bits for
+// negatives).
+struct StateCheck {
+    unsigned long long max_read;  // largest start / close index the LOOP walk reads
+    unsigned long long max_key;   // largest cell key (no u32 wrapping)
+    uint32_t r_min_key;
+    uint32_t r_max_key;
+    uint32_t bad;                 // spheres with a non-finite float or f16 half
+    uint32_t pad;
+};
+
+hipError_t launch_state_check(const PhysConsts &c, const float4 *pos, const uint4 *vfme,
+                              uint32_t *key_out, uint32_t n, StateCheck *out, hipStream_t stream);
 
 // ---- frame path ------------------------------------------------------------
 constexpr uint32_t kMaxRows = 4096;     // G <= 4096 (G^2 <= 2^24)
@@ -261,6 +278,15 @@ namespace mb {
 
 // Host helpers shared by the C ABI files.
 PhysConsts compile_consts(const mb_settings &s);
+// Cell coordinate u32(x + HALF_MAP_SIZE) / GRID_SIZE (not clamped).
+uint64_t host_cell(const PhysConsts &c, float x);
+// Largest cell coordinate a LOOP frame's output can have, for radii in
+// [r_min, r_max] (solve_cube_constraint's bound); >= G when outputs can
+// leave the grid.
+uint64_t loop_cell_bound(const PhysConsts &c, float r_min, float r_max);
+// Largest start / close index the neighbour walk reads for cells in
+// [0, cmax]^2 of a G-column grid.
+uint64_t walk_read_bound(uint64_t g, uint64_t cmax);
 // Stores `msg` as the calling thread's last error and returns `code`.
 int set_error(int code, const std::string &msg);
 

@@ -456,14 +456,45 @@ __device__ __forceinline__ uint32_t pack_half2(float lo, float hi)
     return (uint32_t)a | ((uint32_t)b << 16);
 }
 
-// get_collision_grid_id (physics.wgsl:107-109), clamped to the grid.
+// get_collision_grid_id (physics.wgsl:107-109), not clamped: a position past
+// the map's +x / +z edge has a coordinate >= G, and its key cz * G + cx
+// (u32, wrapping) aliases a cell of a later row, exactly as in the shader; one
+// past the -x / -z edge saturates to 0.
 __device__ __forceinline__ void cell_of(const PhysConsts &c, float x, float z, uint32_t &cx,
                                         uint32_t &cz)
 {
-    const uint32_t ix = fast_div(f32_to_u32_sat(x + c.half_map_size), c.grid_size_div);
-    const uint32_t iz = fast_div(f32_to_u32_sat(z + c.half_map_size), c.grid_size_div);
-    cx = min(ix, c.grids - 1u);
-    cz = min(iz, c.grids - 1u);
+    cx = fast_div(f32_to_u32_sat(x + c.half_map_size), c.grid_size_div);
+    cz = fast_div(f32_to_u32_sat(z + c.half_map_size), c.grid_size_div);
+}
+
+// The same for the frame path, whose row buckets cover the grid's G rows.  The
+// host sends every state whose cells can leave the grid (an upload with a
+// sphere outside it, a radius that lets a wall-clamped position reach past the
+// edge, the frame after a grid change) through the stage kernels instead, so
+// on this path the min() never changes a cell: it only keeps a routing mistake
+// from writing outside the buckets.
+__device__ __forceinline__ void cell_in_grid(const PhysConsts &c, float x, float z, uint32_t &cx,
+                                             uint32_t &cz)
+{
+    cell_of(c, x, z, cx, cz);
+    cx = min(cx, c.grids - 1u);
+    cz = min(cz, c.grids - 1u);
+}
+
+// start / close reads of the stage kernels.  The arrays hold `cells` entries
+// (the reference's N); only keys of positions far outside the map index past
+// them (WGSL returns an adapter-defined value there), and such a read is an
+// empty cell here.  The host rejects every state whose walk would reach one.
+__device__ __forceinline__ uint32_t rd_start(const uint32_t *__restrict__ start, uint32_t i,
+                                             uint32_t cells)
+{
+    return i < cells ? start[i] : kInvalid;
+}
+
+__device__ __forceinline__ uint32_t rd_close(const uint32_t *__restrict__ close, uint32_t i,
+                                             uint32_t cells)
+{
+    return i < cells ? close[i] : 0u;
 }
 
 struct Agent {
@@ -605,19 +636,18 @@ __device__ __forceinline__ void solve_sphere(const PhysConsts &c, Agent &a, floa
 
 template <bool SPRING>
 __device__ __forceinline__ void collide_range(const PhysConsts &c, Agent &a, uint32_t first,
-                                              uint32_t last, uint32_t skip,
+                                              uint32_t last, uint32_t skip, uint32_t n,
                                               const uint32_t *__restrict__ perm,
                                               const float4 *__restrict__ pos_i,
                                               const uint4 *__restrict__ vfme_i)
 {
     // `for (var i = first; i <= last; i++)`; first == INVALID means empty.
-    if (first == kInvalid) return;
-    for (uint32_t i = first; i <= last; ++i) {
+    // Slots past N (out of the reference's bounds, see rd_start) end it.
+    for (uint32_t i = first; i <= last && i < n; ++i) {
         if (i != skip) {
             const uint32_t bo = perm[i];
             solve_sphere<SPRING>(c, a, pos_i[bo], vfme_i, bo);
         }
-        if (i == 0xFFFFFFFEu) break;
     }
 }
 
@@ -625,34 +655,34 @@ __device__ __forceinline__ void collide_range(const PhysConsts &c, Agent &a, uin
 template <bool SPRING>
 __device__ __forceinline__ void handle_row(const PhysConsts &c, Agent &a, uint32_t cx,
                                            uint32_t base, const uint32_t *__restrict__ start,
-                                           const uint32_t *__restrict__ close,
-                                           const uint32_t *__restrict__ perm,
+                                           const uint32_t *__restrict__ close, uint32_t cells,
+                                           uint32_t n, const uint32_t *__restrict__ perm,
                                            const float4 *__restrict__ pos_i,
                                            const uint4 *__restrict__ vfme_i)
 {
     uint32_t first = kInvalid, last = 0u;
     if (cx > 0u) {
-        const uint32_t st = start[base - 1u];
+        const uint32_t st = rd_start(start, base - 1u, cells);
         if (st != kInvalid) {
             first = st;
-            last = close[base - 1u];
+            last = rd_close(close, base - 1u, cells);
         }
     }
     {
-        const uint32_t st = start[base];
+        const uint32_t st = rd_start(start, base, cells);
         if (st != kInvalid) {
             if (first == kInvalid) first = st;
-            last = close[base];
+            last = rd_close(close, base, cells);
         }
     }
     if (cx < c.grids - 1u) {
-        const uint32_t st = start[base + 1u];
+        const uint32_t st = rd_start(start, base + 1u, cells);
         if (st != kInvalid) {
             if (first == kInvalid) first = st;
-            last = close[base + 1u];
+            last = rd_close(close, base + 1u, cells);
         }
     }
-    collide_range<SPRING>(c, a, first, last, kInvalid, perm, pos_i, vfme_i);
+    collide_range<SPRING>(c, a, first, last, kInvalid, n, perm, pos_i, vfme_i);
 }
 
 // solve_cube_constraint (physics.wgsl:269-298).
@@ -750,8 +780,8 @@ __device__ __forceinline__ void jitter(Agent &a, uint32_t s, float ox, float oy,
 }
 
 // physics.wgsl:80-88 after the collisions: walls, f16 clamp, pack, store,
-// next cell key.  Returns the key.
-template <bool SPRING>
+// next cell key (IN_GRID: the frame path's cell_in_grid).  Returns the key.
+template <bool SPRING, bool IN_GRID>
 __device__ __forceinline__ uint32_t finish_agent(const PhysConsts &c, Agent &a, uint32_t s,
                                                  float4 *__restrict__ pos_o,
                                                  uint4 *__restrict__ vfme_o,
@@ -771,7 +801,10 @@ __device__ __forceinline__ uint32_t finish_agent(const PhysConsts &c, Agent &a, 
     vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
                            pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
     uint32_t nx, nz;
-    cell_of(c, a.px, a.pz, nx, nz);
+    if (IN_GRID)
+        cell_in_grid(c, a.px, a.pz, nx, nz);
+    else
+        cell_of(c, a.px, a.pz, nx, nz);
     const uint32_t key = nz * c.grids + nx;
     key_out[s] = key;
     return key;
@@ -783,7 +816,7 @@ __device__ __forceinline__ uint32_t finish_agent(const PhysConsts &c, Agent &a, 
 template <bool SPRING>
 __global__ __launch_bounds__(256) void physics_loop_kernel(
     PhysConsts c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ start,
-    const uint32_t *__restrict__ close, uint32_t *__restrict__ key_out,
+    const uint32_t *__restrict__ close, uint32_t cells, uint32_t *__restrict__ key_out,
     const float4 *__restrict__ pos_i, const uint4 *__restrict__ vfme_i,
     float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o, uint32_t n)
 {
@@ -797,23 +830,24 @@ __global__ __launch_bounds__(256) void physics_loop_kernel(
     verlet(c, a);
     const float ox = a.px, oy = a.py, oz = a.pz;
     {
-        uint32_t first = start[k];
-        uint32_t last = close[k];
+        uint32_t first = rd_start(start, k, cells);
+        uint32_t last = rd_close(close, k, cells);
         if (cx > 0u) {
-            const uint32_t st = start[k - 1u];
+            const uint32_t st = rd_start(start, k - 1u, cells);
             if (st != kInvalid) first = st;
         }
         if (cx < c.grids - 1u) {
-            const uint32_t st = start[k + 1u];
-            if (st != kInvalid) last = close[k + 1u];
+            const uint32_t st = rd_start(start, k + 1u, cells);
+            if (st != kInvalid) last = rd_close(close, k + 1u, cells);
         }
-        collide_range<SPRING>(c, a, first, last, s, perm, pos_i, vfme_i);
+        collide_range<SPRING>(c, a, first, last, s, n, perm, pos_i, vfme_i);
     }
-    if (cz > 0u) handle_row<SPRING>(c, a, cx, k - c.grids, start, close, perm, pos_i, vfme_i);
+    if (cz > 0u)
+        handle_row<SPRING>(c, a, cx, k - c.grids, start, close, cells, n, perm, pos_i, vfme_i);
     if (cz < c.grids - 1u)
-        handle_row<SPRING>(c, a, cx, k + c.grids, start, close, perm, pos_i, vfme_i);
+        handle_row<SPRING>(c, a, cx, k + c.grids, start, close, cells, n, perm, pos_i, vfme_i);
     jitter(a, s, ox, oy, oz);
-    (void)finish_agent<SPRING>(c, a, s, pos_o, vfme_o, key_out);
+    (void)finish_agent<SPRING, false>(c, a, s, pos_o, vfme_o, key_out);
 }
 
 // main() with INIT_PIPELINE (physics.wgsl:91-96).
@@ -836,9 +870,9 @@ __global__ __launch_bounds__(256) void physics_init_kernel(
 }
 
 hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
-                          const uint32_t *start, const uint32_t *close, uint32_t *key_out,
-                          const float4 *pos_i, const uint4 *vfme_i, float4 *pos_o,
-                          uint4 *vfme_o, uint32_t n, hipStream_t stream)
+                          const uint32_t *start, const uint32_t *close, uint32_t cells,
+                          uint32_t *key_out, const float4 *pos_i, const uint4 *vfme_i,
+                          float4 *pos_o, uint4 *vfme_o, uint32_t n, hipStream_t stream)
 {
     if (n == 0) return hipSuccess;
     const dim3 grid((n + 255u) / 256u);
@@ -847,10 +881,10 @@ hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
                            pos_i, vfme_i, pos_o, vfme_o, n);
     } else if (c.enable_spring) {
         hipLaunchKernelGGL(physics_loop_kernel<true>, grid, dim3(256), 0, stream, c, perm, start,
-                           close, key_out, pos_i, vfme_i, pos_o, vfme_o, n);
+                           close, cells, key_out, pos_i, vfme_i, pos_o, vfme_o, n);
     } else {
         hipLaunchKernelGGL(physics_loop_kernel<false>, grid, dim3(256), 0, stream, c, perm,
-                           start, close, key_out, pos_i, vfme_i, pos_o, vfme_o, n);
+                           start, close, cells, key_out, pos_i, vfme_i, pos_o, vfme_o, n);
     }
     return hipGetLastError();
 }
@@ -1424,7 +1458,7 @@ __device__ __forceinline__ void physics_step_body(
     else
         walk_candidates<SPRING, false>(c, a, s, rg, s_pos, s_vm, perm, pos_i, vfme_i);
     jitter(a, s, ox, oy, oz);
-    const uint32_t next = finish_agent<SPRING>(c, a, s, pos_o, vfme_o, keys);
+    const uint32_t next = finish_agent<SPRING, true>(c, a, s, pos_o, vfme_o, keys);
     const uint32_t nz = fast_div(next, c.grids_div);
     block_file(s_file, rb, rb.fill + set_out * G * kFillStride,
                rb.spill_count + set_out * kFillStride, fast_div(kfirst, c.grids_div), nz,
@@ -1458,7 +1492,7 @@ __device__ __forceinline__ void physics_init_body(
     const float4 p = pos_i[s];
     const Agent a = agent_from(p, vfme_i[s]);
     uint32_t cx, cz;
-    cell_of(c, a.px, a.pz, cx, cz);
+    cell_in_grid(c, a.px, a.pz, cx, cz);
     pos_o[s] = p;
     vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
                            pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
@@ -1515,7 +1549,7 @@ __global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *
     if (s >= n) return;
     const float4 p = pos[s];
     uint32_t cx, cz;
-    cell_of(c, p.x, p.z, cx, cz);
+    cell_in_grid(c, p.x, p.z, cx, cz);
     key_out[s] = cz * c.grids + cx;
     bucket_file<kSpread>(rb, rb.spread, rb.spill_count + set * kFillStride, cz, cx, s);
 }
@@ -1725,6 +1759,77 @@ hipError_t launch_batch_frame(const BatchPlan &p, uint32_t k, bool from_counter,
                            stream, p.scenes, p.blk_spring, k, counter);
     if (from_counter)
         hipLaunchKernelGGL(batch_advance_kernel, dim3(1), dim3(64), 0, stream, p.frame_counter);
+    return hipGetLastError();
+}
+
+// What the host needs to route a state (mb_api.hip): the largest cell key,
+// the largest start / close index the LOOP walk of physics.wgsl:119-160 reads
+// (k and k + 1, or k + G and k + G + 1 when a row lies above), both without
+// u32 wrapping; the smallest and largest radius; and the number of spheres
+// with a non-finite position, radius or f16 half.  With key_out, also writes
+// each sphere's cell key (the shader's, wrapping).
+__device__ __forceinline__ uint32_t float_order_key(float f)
+{
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ bool finite_word_halves(uint32_t w)
+{
+    return ((w >> 10) & 0x1Fu) != 0x1Fu && ((w >> 26) & 0x1Fu) != 0x1Fu;
+}
+
+__global__ __launch_bounds__(256) void state_check_kernel(PhysConsts c,
+                                                          const float4 *__restrict__ pos,
+                                                          const uint4 *__restrict__ vfme,
+                                                          uint32_t *__restrict__ key_out,
+                                                          uint32_t n, StateCheck *__restrict__ out)
+{
+    const uint64_t G = c.grids;
+    unsigned long long max_read = 0, max_key = 0;
+    uint32_t rmin = 0xFFFFFFFFu, rmax = 0u, bad = 0u;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        const float4 p = pos[i];
+        const uint4 q = vfme[i];
+        const bool fin = isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(p.w) &&
+                         finite_word_halves(q.x) && finite_word_halves(q.y) &&
+                         finite_word_halves(q.z) && finite_word_halves(q.w);
+        bad += fin ? 0u : 1u;
+        uint32_t cx, cz;
+        cell_of(c, p.x, p.z, cx, cz);
+        const uint64_t k = (uint64_t)cz * G + cx;
+        const uint64_t right = (uint64_t)cx + 1u < G ? 1u : 0u;
+        const uint64_t rd = (uint64_t)cz + 1u < G ? k + G + right : k + right;
+        max_key = max(max_key, (unsigned long long)k);
+        max_read = max(max_read, (unsigned long long)rd);
+        const uint32_t ok = float_order_key(p.w);
+        rmin = min(rmin, ok);
+        rmax = max(rmax, ok);
+        if (key_out) key_out[i] = (uint32_t)k;
+    }
+    for (uint32_t off = 32; off > 0; off >>= 1) {
+        max_read = max(max_read, (unsigned long long)__shfl_xor(max_read, off, 64));
+        max_key = max(max_key, (unsigned long long)__shfl_xor(max_key, off, 64));
+        rmin = min(rmin, (uint32_t)__shfl_xor(rmin, off, 64));
+        rmax = max(rmax, (uint32_t)__shfl_xor(rmax, off, 64));
+        bad += (uint32_t)__shfl_xor(bad, off, 64);
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        atomicMax(&out->max_read, max_read);
+        atomicMax(&out->max_key, max_key);
+        atomicMin(&out->r_min_key, rmin);
+        atomicMax(&out->r_max_key, rmax);
+        if (bad) atomicAdd(&out->bad, bad);
+    }
+}
+
+hipError_t launch_state_check(const PhysConsts &c, const float4 *pos, const uint4 *vfme,
+                              uint32_t *key_out, uint32_t n, StateCheck *out, hipStream_t stream)
+{
+    if (n == 0) return hipSuccess;
+    const uint32_t blocks = std::min<uint32_t>((n + 255u) / 256u, 2048u);
+    hipLaunchKernelGGL(state_check_kernel, dim3(blocks), dim3(256), 0, stream, c, pos, vfme,
+                       key_out, n, out);
     return hipGetLastError();
 }
 

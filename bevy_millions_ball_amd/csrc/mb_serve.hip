@@ -717,6 +717,25 @@ int mb_server_submit(mb_server *s, const mb_settings *settings, const float *pos
         if (!finite_half(vfme[i] & 0xFFFFu) || !finite_half(vfme[i] >> 16))
             return fail(MB_ERR_INVALID_ARGUMENT, "non-finite half in packed_vfme at word " + std::to_string(i));
     }
+    {
+        // The batched frame runs the frame path only, whose buckets cover the
+        // grid: every sphere's cell must lie in it, and the radii must keep
+        // wall-clamped positions there (the single-scene handle runs other
+        // states through the stage kernels).
+        const mb::PhysConsts c = mb::compile_consts(*settings);
+        float r_min = pos[3], r_max = pos[3];
+        for (uint32_t i = 0; i < n; ++i) {
+            const float *q = pos + 4u * (size_t)i;
+            if (mb::host_cell(c, q[0]) >= g || mb::host_cell(c, q[2]) >= g)
+                return fail(MB_ERR_INVALID_ARGUMENT,
+                            "sphere " + std::to_string(i) + " lies outside the grid (serving needs every cell in it)");
+            r_min = std::min(r_min, q[3]);
+            r_max = std::max(r_max, q[3]);
+        }
+        if (mb::loop_cell_bound(c, r_min, r_max) >= g)
+            return fail(MB_ERR_INVALID_ARGUMENT,
+                        "the radii let wall-clamped positions leave the grid (serving needs every cell in it)");
+    }
     std::unique_ptr<Request> r(new (std::nothrow) Request());
     if (!r) return fail(MB_ERR_OUT_OF_MEMORY, "host allocation failed");
     r->id = s->next_id++;

@@ -290,36 +290,184 @@ def test_settings_change_and_disable(gpu, oracle):
         assert g.time() == t
 
 
-@pytest.mark.parametrize("split", [1, 4])
-def test_regrid_rekeys_like_a_fresh_upload(gpu, oracle, split):
-    """A new grid mid-run re-keys the current state (its INIT-order filing
-    counts in the spread set; after frame 0 the counts are copied into the
-    set the next frame sorts).  Every LOOP frame after that must equal the
-    oracle's LOOP frames on the same state uploaded fresh (INIT only copies
-    it and keys it), frame by frame."""
+@pytest.mark.parametrize("split,grids,grid_size", [
+    (1, 32, 2),    # same map, coarser cells; right after INIT
+    (4, 32, 2),
+    (3, 32, 1),    # a smaller map: the state lies partly outside it (keys alias later rows)
+    (5, 90, 1),    # a larger map with finer cells, G not a power of two
+])
+def test_regrid_matches_the_reference_frames(gpu, oracle, split, grids, grid_size):
+    """A live grid change (the slider of main.rs:190-200, mod.rs:808-837
+    recompiles the pipelines and carries on with the buffers as they are):
+    the frame after it sorts the keys the last frame wrote under the old
+    grid and walks start / close with the new grid's cells, stale close words
+    included; the frames after it are ordinary.  Frame by frame against the
+    oracle, which keeps the reference's buffers."""
     cfg = replace(CONFIGS["plain_65k"], number_of_agents=8192, grids=64)
-    ps = cfg.settings()
-    pos, vf = generate(cfg, seed=11)
-    with Physics(ps) as g:
-        g.upload_state(pos, vf)
-        g.run(split)
-        p_mid, v_mid = g.download_state()
-        # same map (G * grid_size = 64), coarser cells
-        ps2 = replace(ps, number_of_grids_one_dimension=32, grid_size=2)
+    print(f"[regrid] split={split} -> G={grids} gs={grid_size}", flush=True)
+    g, r = make_pair(oracle, cfg, seed=11)
+    with g:
+        compare_frames(g, r, split)
+        ps2 = replace(cfg.settings(), number_of_grids_one_dimension=grids, grid_size=grid_size)
         g.set_settings(ps2)
-        r = oracle.World(ps2)
-        r.upload(p_mid, v_mid)
-        r.step(1)  # INIT
-        for f in range(4):
+        r.set_settings(ps2)
+        compare_frames(g, r, 6)
+    r.close()
+
+
+def demo_settings():
+    """main.rs:42-49 over the PhysicsPlugin defaults (mod.rs:87-109)."""
+    return PhysicsSettings(number_of_grids_one_dimension=64, half_map_height=8, e_agent=0.8,
+                           e_envir=0.8, _max_number_of_agents=256 * 256)
+
+
+def test_reference_demo_scene_bit_exact(gpu, oracle):
+    """The reference demo: 65,536 spheres on a 64 x 64 grid of cells of size 8
+    (a +-256 map) starting from the lattice of mod.rs:173-196, which reaches
+    |x| = 510, with the ChaCha8 velocities of mod.rs:199-212.  INIT keys cells
+    past the grid (u32(x + 256) / 8 up to 95: keys alias later rows, up to
+    6175); INIT and the frame that sorts those keys run on the stage kernels
+    in the reference's buffer roles, then the frame path takes over."""
+    from bevy_millions_ball_amd.scenes import reference_lattice
+
+    ps = demo_settings()
+    pos, vf = reference_lattice(65536)
+    g = Physics(ps)
+    g.upload_state(pos, vf)
+    r = oracle.World(ps)
+    r.upload(pos, vf)
+    with g:
+        compare_frames(g, r, 1)
+        keys = g.debug()["next_keys"]
+        assert keys.max() == 6175 and (keys >= 64 * 64).sum() > 0
+        compare_frames(g, r, 40)
+    r.close()
+
+
+def test_upload_rejects_walks_past_the_arrays(gpu):
+    """A lattice of 4096 spheres reaching |x| = 126 in a +-32 map: the walk of
+    a sphere at x = 126 reads cell keys past N = 4096, where the reference
+    reads start / close out of bounds.  The upload is refused, and stepping
+    without an accepted state is an error."""
+    from bevy_millions_ball_amd.scenes import reference_lattice
+
+    n = 64 * 64
+    ps = PhysicsSettings(number_of_grids_one_dimension=64, grid_size=1, half_map_height=8,
+                         e_agent=0.8, e_envir=0.8, _max_number_of_agents=n)
+    pos, vf = reference_lattice(n)
+    with Physics(ps) as g:
+        with pytest.raises(Exception, match="out of bounds"):
+            g.upload_state(pos, vf)
+        with pytest.raises(Exception, match="no accepted state"):
             g.run(1)
-            r.step(1)
-            d, rd = g.debug(), r.debug()
-            assert np.array_equal(d["perm"], rd["perm"]), f"perm differs, frame {f}"
-            assert np.array_equal(d["next_keys"], rd["next_keys"]), f"keys differ, frame {f}"
-            pg, vg = g.download_state()
-            pr, vr = r.download()
-            assert_state_close(pg, vg, pr, vr)
-        r.close()
+        # an in-map state is accepted afterwards
+        cfg = replace(CONFIGS["plain_65k"], number_of_agents=n, grids=64)
+        g.upload_state(*generate(cfg, seed=1))
+        g.run(2)
+
+
+@pytest.mark.parametrize("mode", ["spring_r_agent_1", "tiny_radius"])
+def test_outputs_that_can_leave_the_grid(gpu, oracle, mode):
+    """States whose LOOP outputs can reach past the map's +x / +z edge: spring
+    walls with r_agent = 1 (the top of the demo's slider, main.rs:186) clamp
+    a sphere at HALF_MAP_SIZE itself, and a radius far below the f32 spacing
+    at the edge does the same with rigid walls.  Every frame then runs on the
+    stage kernels; keys >= G^2 appear and must match the oracle's."""
+    rng = np.random.default_rng(31)
+    G, n = 64, 8192
+    hms = G // 2
+    if mode == "spring_r_agent_1":
+        ps = PhysicsSettings(enable_spring=True, number_of_grids_one_dimension=G, grid_size=1,
+                             half_map_height=8, k_agent=500.0, c_agent=5.0, r_agent=1.0,
+                             e_agent=0.8, e_envir=0.8, time_delta=1 / 120, _max_number_of_agents=n)
+        radius = 0.5
+    else:
+        ps = PhysicsSettings(number_of_grids_one_dimension=G, grid_size=1, half_map_height=8,
+                             e_agent=0.8, e_envir=0.8, time_delta=1 / 60, _max_number_of_agents=n)
+        radius = 1e-7
+    pos = np.empty((n, 4), np.float32)
+    pos[:, 0] = rng.uniform(-hms + 0.5, hms - 0.5, n)
+    pos[:, 2] = rng.uniform(-hms + 0.5, hms - 0.5, n)
+    pos[:, 1] = rng.uniform(0.5, 15.5, n)
+    pos[:, 3] = radius
+    vel = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    vf = pack_vfme(vel, np.zeros((n, 3), np.float32), np.ones(n, np.float32))
+    g = Physics(ps)
+    g.upload_state(pos, vf)
+    r = oracle.World(ps)
+    r.upload(pos, vf)
+    past = 0
+    with g:
+        for _ in range(12):
+            compare_frames(g, r, 1)
+            past += int((g.debug()["next_keys"] >= G * G).sum())
+    r.close()
+    assert past > 0
+
+
+def test_checkpoint_of_an_out_of_grid_state(gpu):
+    """A checkpoint of the demo's INIT frame holds a state outside the grid:
+    loading it keys the state as the reference would (unclamped) and the
+    next frame sorts those keys on the stage kernels, so resuming equals the
+    uninterrupted run."""
+    from bevy_millions_ball_amd.scenes import reference_lattice
+
+    ps = demo_settings()
+    pos, vf = reference_lattice(65536)
+    with Physics(ps) as a:
+        a.upload_state(pos, vf)
+        a.run(1)
+        ck = a.save_checkpoint()
+        a.run(5)
+        want = a.download_state(), a.debug()["next_keys"]
+    with Physics(ps) as b:
+        b.load_checkpoint(ck)
+        b.run(5)
+        got = b.download_state(), b.debug()["next_keys"]
+    assert np.array_equal(got[0][0].view(np.uint32), want[0][0].view(np.uint32))
+    assert np.array_equal(got[0][1], want[0][1])
+    assert np.array_equal(got[1], want[1])
+
+
+def checkpoint_state(ck: bytes, n: int):
+    head = 4 + 4 + 60 + 8 + 4
+    pos = np.frombuffer(ck, np.float32, n * 4, head).reshape(n, 4)
+    vf = np.frombuffer(ck, np.uint32, n * 4, head + n * 16).reshape(n, 4)
+    return pos, vf
+
+
+def test_synchronous_calls_wait_for_the_callers_stream(gpu, oracle):
+    """Frames enqueued on a side stream (a torch stream does not synchronise
+    with the null stream), then at once a checkpoint, a grid change and an
+    autotune: each must act on the state those frames leave, not on an
+    earlier one."""
+    import torch
+
+    cfg = CONFIGS["uniform_1m"]  # ~50 us per frame: a missing wait would show
+    n = cfg.number_of_agents
+    side = torch.cuda.Stream()
+    g, r = make_pair(oracle, cfg, seed=4)
+    with g:
+        g.run(4, stream=side)
+        ck = g.save_checkpoint()
+        r.step(4)
+        pr, vr = r.download()
+        pc, vc = checkpoint_state(ck, n)
+        assert np.array_equal(pc.view(np.uint32), pr.view(np.uint32)) and np.array_equal(vc, vr)
+        g.run(3, stream=side)
+        r.step(3)
+        ps2 = replace(cfg.settings(), number_of_grids_one_dimension=512, grid_size=2)
+        g.set_settings(ps2)
+        r.set_settings(ps2)
+        g.run(2, stream=side)
+        r.step(2)
+        g.autotune(1)
+        side.synchronize()
+        pg, vg = g.download_state()
+        pr, vr = r.download()
+        assert_state_close(pg, vg, pr, vr)
+        assert np.array_equal(g.debug()["next_keys"], r.debug()["next_keys"])
+    r.close()
 
 
 def test_edge_cases_tiny_worlds(gpu, oracle):
@@ -508,23 +656,6 @@ def test_checkpoint_rejects_other_sizes(gpu):
     with Physics(other.settings()) as b:
         with pytest.raises(Exception, match="4096"):
             b.load_checkpoint(ck)
-
-
-def test_out_of_box_upload_is_clamped_like_oracle(gpu, oracle):
-    """The reference demo's own start lattice lies partly outside its box
-    (main.rs:43-44 with mod.rs:173-196); cell ids are clamped to the grid."""
-    from bevy_millions_ball_amd.scenes import reference_lattice
-
-    n = 64 * 64
-    ps = PhysicsSettings(number_of_grids_one_dimension=64, grid_size=1, half_map_height=8,
-                         e_agent=0.8, e_envir=0.8, _max_number_of_agents=n)
-    pos, vf = reference_lattice(n)
-    g = Physics(ps)
-    g.upload_state(pos, vf)
-    r = oracle.World(ps)
-    r.upload(pos, vf)
-    with g:
-        compare_frames(g, r, 4)
 
 
 @pytest.mark.parametrize("grids,grid_size,n,mass_mode", [

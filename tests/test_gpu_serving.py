@@ -217,3 +217,23 @@ def test_frame_times_and_status(gpu):
             srv.collect(rid + 99)
         res = srv.collect(rid)
         assert res.perm is None
+
+
+def test_submit_rejects_states_outside_the_grid(gpu):
+    """The batched frame runs the frame path only: a scene with a sphere
+    outside its grid, or radii that let a wall-clamped position leave it, is
+    refused at submit (the single-scene handle runs such states through the
+    stage kernels instead)."""
+    st, pos, vf, _ = scene(4096, 2, 12)
+    out = pos.copy()
+    out[7, 0] = 40.0                       # map is +-32: column 72
+    tiny = pos.copy()
+    tiny[:, 3] = 1e-7                      # the wall clamp lands on the +32 edge
+    with Server() as srv:
+        with pytest.raises(Exception, match="outside the grid"):
+            srv.submit(st, out, vf, 2)
+        with pytest.raises(Exception, match="leave the grid"):
+            srv.submit(st, tiny, vf, 2)
+        rid = srv.submit(st, pos, vf, 2)
+        srv.drain()
+        srv.collect(rid)
