@@ -134,6 +134,10 @@ SIGNATURES = {
         C.c_int,
         [_P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P],
     ),
+    "mb_radix_sort_static": (
+        C.c_int,
+        [_P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P],
+    ),
     "mb_spatial_hashing": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P]),
     "mb_physics_run": (
         C.c_int,

@@ -15,7 +15,7 @@ from pathlib import Path
 PKG_DIR = Path(__file__).resolve().parent
 CSRC = PKG_DIR / "csrc"
 LIB_PATH = PKG_DIR / "libmillions_ball.so"
-SOURCES = ["mb_kernels.hip", "mb_api.hip", "mb_serve.hip"]
+SOURCES = ["mb_kernels.hip", "mb_api.hip", "mb_serve.hip", "mb_static_sort.hip"]
 ARCH = os.environ.get("MB_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -213,6 +213,19 @@ int mb_radix_sort(void *workspace, const uint32_t *keys_in, const uint32_t *vals
                   uint32_t *keys_out, uint32_t *vals_out, uint32_t number_of_keys,
                   uint32_t pass_begin, uint32_t pass_end, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mb_radix_sort with the library's own static device
+ * workspace instead of a `workspace` argument, same kernels and results.
+ * At most MB_RADIX_SORT_STATIC_MAX_KEYS keys (more: MB_ERR_INVALID_ARGUMENT).
+ * The workspace (8.25 MiB) exists once per GPU the library runs on, is held
+ * until the process ends and is shared by every call on that GPU: calls that
+ * overlap on one GPU, on any streams, race.  Allocates nothing. */
+#define MB_RADIX_SORT_STATIC_MAX_KEYS (1u << 20)
+int mb_radix_sort_static(const uint32_t *keys_in, const uint32_t *vals_in, uint32_t *keys_out,
+                         uint32_t *vals_out, uint32_t number_of_keys, uint32_t pass_begin,
+                         uint32_t pass_end, void *stream);
+
 /* spatial_hashing::run(.., number_of_keys, read_from_even)
  * (spatial_hashing.rs:191-243): start[0..zero_count) = MB_INVALID_INDEX, then
  * start[k] / close[k] for the first / last sorted index of every key k.  Keys
