@@ -100,18 +100,29 @@ int mb_physics_destroy(mb_physics *physics);
 
 /* Live settings change (ResMut<PhysicsSettings> -> PhysicsNode::update,
  * mod.rs:808-837).  max_number_of_agents cannot change (the buffers are sized
- * once, as in the reference).  `time` keeps counting; a change of the grid
- * (count, cell size) re-keys the current state before the next frame. */
+ * once, as in the reference).  `time` keeps counting.  After a change of the
+ * grid (count, cell size) the next frame does what the reference's does: it
+ * sorts the keys the last frame wrote under the old grid and walks the ranges
+ * with the new grid's cells.  Waits for the stream of the last stream-ordered
+ * call first.  Rejects (settings unchanged) a change under which the current
+ * state's neighbour walk would read start / close past N (see
+ * mb_physics_upload_state). */
 int mb_physics_set_settings(mb_physics *physics, const mb_settings *settings);
 int mb_physics_get_settings(const mb_physics *physics, mb_settings *settings);
 
 /* Writes N spheres of host state into the EVE state buffers and zeroes the
  * sort keys (the buffers' state at PhysicsInnerPlugin::build), then resets
- * time to -1 so that the next frame runs the INIT pipeline.  Rejects
- * non-finite positions, radii or velocities. */
+ * time to -1 so that the next frame runs the INIT pipeline.  Cells are the
+ * shader's, unclamped (physics.wgsl:107-117): a sphere past the map's +x / +z
+ * edge has a key that aliases a cell of a later row, as in the reference.
+ * Rejects non-finite positions, radii or velocities, and a state for which
+ * the reference would read its N-entry start / close arrays out of bounds: a
+ * sphere whose neighbour walk reaches cell key N, or radii that let a
+ * wall-clamped position reach one later.  After a rejected upload the handle
+ * holds no state and mb_physics_step returns MB_ERR_STATE. */
 int mb_physics_upload_state(mb_physics *physics, const float *position_with_scale,
                             const uint32_t *packed_vfme, void *stream);
-/* Same from device memory (N float4 + N uint4, no validation). */
+/* Same from device memory (N float4 + N uint4), with the same checks. */
 int mb_physics_upload_state_device(mb_physics *physics,
                                    const void *position_with_scale,
                                    const void *packed_vfme, void *stream);
@@ -155,7 +166,10 @@ int mb_physics_autotune(mb_physics *physics, uint32_t frames_per_trial, mb_tunin
 size_t mb_checkpoint_size(uint32_t number_of_agents);
 int mb_physics_save_checkpoint(const mb_physics *physics, void *buffer, size_t size);
 /* The handle's max_number_of_agents must equal the checkpoint's N; its
- * settings are replaced by the checkpoint's (which may change the grid). */
+ * settings are replaced by the checkpoint's (which may change the grid).
+ * The state is checked as mb_physics_upload_state checks one.  A grid change
+ * right after a load reads stale close words the checkpoint does not carry
+ * (the permutation of the frame before). */
 int mb_physics_load_checkpoint(mb_physics *physics, const void *buffer, size_t size);
 /* Reads a checkpoint's header without a device: settings, time and N. */
 int mb_checkpoint_inspect(const void *buffer, size_t size, mb_settings *settings, int64_t *time,
@@ -235,7 +249,9 @@ int mb_spatial_hashing(const uint32_t *sorted_keys, uint32_t *start, uint32_t *c
 
 /* physics::run(.., number_of_agents) (mod.rs:915-956) with the bind group of
  * one parity (mod.rs:659-705) spelled out: init != 0 selects INIT_PIPELINE.
- * start / close must cover number_of_grids_one_dimension^2 cells. */
+ * start / close hold max(number_of_agents, number_of_grids_one_dimension^2)
+ * entries (the reference's N whenever G^2 <= N); a read past them (only a
+ * sphere far outside the map reaches one) is an empty cell. */
 int mb_physics_run(const mb_settings *settings, int init, const uint32_t *perm,
                    const uint32_t *start, const uint32_t *close, uint32_t *key_out,
                    const void *position_with_scale_in, const void *packed_vfme_in,
