@@ -58,28 +58,48 @@ def read_traffic(name: str):
     return None
 
 
-def cpu_baseline(cfg, pos, vf, budget_s: float, threads: int) -> dict:
+def cpu_frames(cfg, pos, vf, budget_s: float, threads: int):
+    """LOOP frames of the oracle after INIT for about budget_s seconds:
+    (frames, seconds, per-frame seconds of [sort, spatial hashing, physics])."""
     from oracle import mb_oracle
 
     w = mb_oracle.World(cfg.settings())
     w.upload(pos, vf)
     w.step(1, threads=threads)  # INIT frame, not counted
+    stages = [0.0, 0.0, 0.0]
     frames = 0
     t0 = time.perf_counter()
     while True:
-        w.step(1, threads=threads)
+        w.step(1, threads=threads, stage_seconds=stages)
         frames += 1
         el = time.perf_counter() - t0
         if el >= budget_s or frames >= 2000:
             break
     w.close()
+    return frames, el, [s / frames for s in stages]
+
+
+def cpu_baseline(cfg, pos, vf, budget_s: float, threads: int) -> dict:
+    """The oracle (the C transcription of the reference's frame) on the host:
+    OpenMP over `threads` cores for budget_s seconds, then one thread for
+    half that, each with its stage split."""
+    frames, el, stages = cpu_frames(cfg, pos, vf, budget_s, threads)
+    f1, el1, stages1 = cpu_frames(cfg, pos, vf, budget_s / 2.0, 1)
+    names = ("sort", "spatial_hashing", "physics")
+    n = cfg.number_of_agents
     return {
-        "value": cfg.number_of_agents * frames / el,
+        "value": n * frames / el,
         "unit": "sphere-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{cfg.name}: {cfg.number_of_agents} spheres, {frames} LOOP frames after INIT, "
+        "sample": f"{cfg.name}: {n} spheres, {frames} LOOP frames after INIT, "
                   f"{el:.1f} s, C transcription with OpenMP",
+        "stage_ms_per_frame": {k: v * 1e3 for k, v in zip(names, stages)},
+        "single_thread": {
+            "value": n * f1 / el1,
+            "sample": f"{f1} LOOP frames after INIT, {el1:.1f} s, one thread",
+            "stage_ms_per_frame": {k: v * 1e3 for k, v in zip(names, stages1)},
+        },
     }
 
 
@@ -272,7 +292,8 @@ def other_config(name: str, seed: int, steps: int) -> dict:
     return {
         "spheres": n, "grids": cfg.grids, "spring": cfg.enable_spring, "layout": cfg.layout,
         "time_delta": cfg.time_delta, "steps": steps,
-        "tuning": {"scan_margin": tuning[0], "stage_pool": tuning[1]},
+        "tuning": {"scan_margin": tuning[0], "stage_pool": tuning[1],
+                   "row_sort_threads": tuning[2]},
         "value": n * steps / el, "unit": "sphere-steps/s", "ms_per_step": el * 1e3 / steps,
         "stage_ms": {"row_sort": float(stage_ms[0]), "collide_integrate": float(stage_ms[1])},
         "collide_roofline_frac": achieved / HBM_PEAK_GBS,
@@ -287,7 +308,7 @@ def main() -> int:
     ap.add_argument("--config", default="uniform_1m")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--latency-steps", type=int, default=200)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serving-requests", type=int, default=64)
     ap.add_argument("--poisson-requests", type=int, default=200)

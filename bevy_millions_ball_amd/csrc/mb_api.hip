@@ -813,6 +813,7 @@ int mb_physics_get_tuning(const mb_physics *p, mb_tuning *t)
     if (!p || !t) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
     t->scan_margin = p->scan_margin;
     t->stage_pool = p->tuning.big_pool ? mb::kStagePoolBig : mb::kStagePool;
+    t->row_sort_threads = p->tuning.narrow_sort ? 256u : mb::kSortThreads;
     return MB_OK;
 }
 
@@ -823,9 +824,12 @@ int mb_physics_set_tuning(mb_physics *p, const mb_tuning *t)
         return fail(MB_ERR_INVALID_ARGUMENT, "scan_margin must be in (0, 4]");
     if (t->stage_pool != mb::kStagePool && t->stage_pool != mb::kStagePoolBig)
         return fail(MB_ERR_INVALID_ARGUMENT, "stage_pool must be 896 or 1536");
+    if (t->row_sort_threads != 256u && t->row_sort_threads != mb::kSortThreads)
+        return fail(MB_ERR_INVALID_ARGUMENT, "row_sort_threads must be 256 or 512");
     p->scan_margin = t->scan_margin;
     p->consts.scan_margin = t->scan_margin;
     p->tuning.big_pool = t->stage_pool == mb::kStagePoolBig;
+    p->tuning.narrow_sort = t->row_sort_threads == 256u;
     return MB_OK;
 }
 
@@ -840,18 +844,21 @@ int mb_physics_autotune(mb_physics *p, uint32_t frames, mb_tuning *best, double 
     std::vector<char> ck(mb_checkpoint_size(p->n));
     int rc = mb_physics_save_checkpoint(p, ck.data(), ck.size());
     if (rc != MB_OK) return rc;
-    const mb_tuning keep{p->scan_margin, p->tuning.big_pool ? mb::kStagePoolBig : mb::kStagePool};
+    mb_tuning keep;
+    (void)mb_physics_get_tuning(p, &keep);
     const float margins[] = {1.0f, 0.5f, 0.25f};
     const uint32_t pools[] = {mb::kStagePool, mb::kStagePoolBig};
+    const uint32_t sorts[] = {mb::kSortThreads, 256u};
     mb_tuning winner = keep;
     double winner_ms = 0.0;
     bool have = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MB_HIP(hipEventCreate(&e0));
     MB_HIP(hipEventCreate(&e1));
+    for (uint32_t sort : sorts)
     for (uint32_t pool : pools) {
         for (float margin : margins) {
-            const mb_tuning t{margin, pool};
+            const mb_tuning t{margin, pool, sort};
             rc = mb_physics_load_checkpoint(p, ck.data(), ck.size());
             if (rc == MB_OK) rc = mb_physics_set_tuning(p, &t);
             if (rc == MB_OK && p->time < 0) rc = mb_physics_step(p, 1, nullptr);  // INIT

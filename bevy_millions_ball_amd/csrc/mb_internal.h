@@ -202,7 +202,8 @@ struct FrameBuffers {
 
 // Launch choices of a frame that do not change results.
 struct FrameTuning {
-    bool big_pool = false;   // collide stages kStagePoolBig slots
+    bool big_pool = false;     // collide stages kStagePoolBig slots
+    bool narrow_sort = false;  // row sort blocks of 256 threads keeping kRowCapNarrow spheres in LDS
 };
 
 // Enqueues frame `time`: INIT at time 0 (offsets, copy, keys, buckets for

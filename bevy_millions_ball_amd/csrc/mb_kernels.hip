@@ -1204,13 +1204,14 @@ __device__ __forceinline__ void row_sort_body(
     if (tid == 0 && r == grids - 1u) off[grids * grids] = n;
 }
 
-__global__ __launch_bounds__(kSortThreads) void row_sort_kernel(
+template <uint32_t NT, uint32_t CAP>
+__global__ __launch_bounds__(NT) void row_sort_kernel(
     RowBuckets rb, uint32_t set, uint32_t spread_in, uint64_t *__restrict__ scratch_pairs,
     uint32_t *__restrict__ scratch_idx, uint32_t grids, uint32_t n,
     uint32_t *__restrict__ perm, uint32_t *__restrict__ off, uint32_t *__restrict__ sorted_keys)
 {
-    row_sort_body<kSortThreads>(rb, set, spread_in != 0u, scratch_pairs, scratch_idx, grids, n,
-                                perm, off, sorted_keys, xcd_block(blockIdx.x, gridDim.x));
+    row_sort_body<NT, CAP>(rb, set, spread_in != 0u, scratch_pairs, scratch_idx, grids, n, perm,
+                           off, sorted_keys, xcd_block(blockIdx.x, gridDim.x));
 }
 
 template <uint32_t NT = kSortThreads, uint32_t CAP = kRowCap>
@@ -1601,9 +1602,15 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time
         return hipGetLastError();
     }
     const uint32_t set = (uint32_t)(time & 1);
-    hipLaunchKernelGGL(row_sort_kernel, dim3(G), dim3(kSortThreads), row_sort_lds(G), stream, b.rb, set,
-                       time == 1 ? 1u : 0u, b.scratch_pairs, b.scratch_idx, G, n, b.perm, b.off,
-                       b.keys_out);
+    if (tuning.narrow_sort)
+        hipLaunchKernelGGL((row_sort_kernel<256, kRowCapNarrow>), dim3(G), dim3(256),
+                           (row_sort_lds<256, kRowCapNarrow>(G)), stream, b.rb, set,
+                           time == 1 ? 1u : 0u, b.scratch_pairs, b.scratch_idx, G, n, b.perm, b.off,
+                           b.keys_out);
+    else
+        hipLaunchKernelGGL((row_sort_kernel<kSortThreads, kRowCap>), dim3(G), dim3(kSortThreads),
+                           row_sort_lds(G), stream, b.rb, set, time == 1 ? 1u : 0u, b.scratch_pairs,
+                           b.scratch_idx, G, n, b.perm, b.off, b.keys_out);
     if ((e = mark(1)) != hipSuccess) return e;
     if (tuning.big_pool)
         launch_collide<kStagePoolBig>(b, c, set ^ 1u, n, stream);

@@ -229,22 +229,24 @@ class Physics:
 
     # -- tuning (launch choices that do not change results) ---------------
     @property
-    def tuning(self) -> Tuple[float, int]:
+    def tuning(self) -> Tuple[float, int, int]:
+        """(scan_margin, stage_pool, row_sort_threads)."""
         t = MbTuning()
         check(lib().mb_physics_get_tuning(self._h, C.byref(t)))
-        return float(t.scan_margin), int(t.stage_pool)
+        return float(t.scan_margin), int(t.stage_pool), int(t.row_sort_threads)
 
-    def set_tuning(self, scan_margin: float, stage_pool: int) -> None:
-        t = MbTuning(float(scan_margin), int(stage_pool))
+    def set_tuning(self, scan_margin: float, stage_pool: int, row_sort_threads: int = 512) -> None:
+        t = MbTuning(float(scan_margin), int(stage_pool), int(row_sort_threads))
         check(lib().mb_physics_set_tuning(self._h, C.byref(t)))
 
-    def autotune(self, frames_per_trial: int = 5) -> Tuple[Tuple[float, int], float]:
+    def autotune(self, frames_per_trial: int = 5) -> Tuple[Tuple[float, int, int], float]:
         """Times every candidate tuning on a copy of the state, keeps the
-        fastest; returns ((scan_margin, stage_pool), ms per frame)."""
+        fastest; returns ((scan_margin, stage_pool, row_sort_threads), ms per
+        frame)."""
         t = MbTuning()
         ms = C.c_double()
         check(lib().mb_physics_autotune(self._h, int(frames_per_trial), C.byref(t), C.byref(ms)))
-        return (float(t.scan_margin), int(t.stage_pool)), ms.value
+        return (float(t.scan_margin), int(t.stage_pool), int(t.row_sort_threads)), ms.value
 
     # -- checkpoints --------------------------------------------------------
     def save_checkpoint(self) -> bytes:

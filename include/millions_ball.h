@@ -137,11 +137,14 @@ int mb_physics_download_state(const mb_physics *physics, float *position_with_sc
  * the exact test; narrower: the scan restarts more often), in (0, 4];
  * stage_pool is the number of neighbour slots a collide block stages in LDS,
  * 896 (7 blocks per CU) or 1536 (4 blocks per CU; fewer blocks of dense
- * scenes fall back to reading neighbours from global memory).  Defaults:
- * 1.0 and 896. */
+ * scenes fall back to reading neighbours from global memory);
+ * row_sort_threads is the width of a row sort block, 512 (rows of up to 4096
+ * spheres sorted in LDS) or 256 (up to 2048; twice as many rows resident).
+ * Defaults: 1.0, 896 and 512. */
 typedef struct mb_tuning {
     float scan_margin;
     uint32_t stage_pool;
+    uint32_t row_sort_threads;
 } mb_tuning;
 
 int mb_physics_get_tuning(const mb_physics *physics, mb_tuning *tuning);

@@ -574,33 +574,45 @@ def test_checkpoint_resume_is_bit_exact(gpu, split):
         assert np.array_equal(res[1], want[1])
 
 
-TUNINGS = [(m, p) for p in (896, 1536) for m in (1.0, 0.5, 0.25)]
+TUNINGS = [(m, p, s) for s in (512, 256) for p in (896, 1536) for m in (1.0, 0.5, 0.25)]
 
 
-@pytest.mark.parametrize("scene", ["uniform", "dense"])
-def test_tuning_does_not_change_results(gpu, scene):
-    """Every tuning choice gives the default's results, bit for bit."""
+@pytest.mark.parametrize("scene", ["uniform", "dense", "clumps"])
+def test_every_tuning_matches_the_oracle(gpu, oracle, scene):
+    """Every point the autotuner can pick (scan margin x staging pool x row
+    sort width, the spring-clumps winner 0.25 / 1536 among them) runs the
+    oracle's frames bit for bit on a uniform scene, a scene with dense rows
+    (bucket spill, rows sorted in global memory, unstaged collide blocks) and
+    a spring-clumps scene."""
     from scene_helpers import dense_row_scene
 
     if scene == "uniform":
         cfg = replace(CONFIGS["uniform_1m"], number_of_agents=20_000, grids=128)
         ps = cfg.settings()
         pos, vf = generate(cfg, seed=9)
+    elif scene == "clumps":
+        cfg = replace(CONFIGS["spring_clumps_1m"], number_of_agents=65_536, grids=256, clumps=8)
+        ps = cfg.settings()
+        pos, vf = generate(cfg, seed=9)
     else:
         ps, pos, vf = dense_row_scene()
-    runs = []
-    for margin, pool in TUNINGS:
+    r = oracle.World(ps)
+    r.upload(pos, vf)
+    r.step(6)
+    pr, vr = r.download()
+    kr = r.debug()["next_keys"]
+    r.close()
+    for margin, pool, sort in TUNINGS:
+        print(f"[tuning] {scene} {margin} {pool} {sort}", flush=True)
         with Physics(ps) as g:
-            g.set_tuning(margin, pool)
-            assert g.tuning == (margin, pool)
+            g.set_tuning(margin, pool, sort)
+            assert g.tuning == (margin, pool, sort)
             g.upload_state(pos, vf)
             g.run(6)
-            p2, v2 = g.download_state()
-            runs.append((p2.view(np.uint32), v2, g.debug()["next_keys"], (margin, pool)))
-    for p2, v2, k2, t in runs[1:]:
-        assert np.array_equal(p2, runs[0][0]), t
-        assert np.array_equal(v2, runs[0][1]), t
-        assert np.array_equal(k2, runs[0][2]), t
+            pg, vg = g.download_state()
+            assert np.array_equal(pg.view(np.uint32), pr.view(np.uint32)), (margin, pool, sort)
+            assert np.array_equal(vg, vr), (margin, pool, sort)
+            assert np.array_equal(g.debug()["next_keys"], kr), (margin, pool, sort)
 
 
 @pytest.mark.parametrize("pool", [896, 1536])
@@ -614,7 +626,7 @@ def test_large_scene_collide_variant_bit_exact(gpu, oracle, pool):
         print(f"[large] {name} n={n} pool={pool}", flush=True)
         g, r = make_pair(oracle, cfg, seed=13)
         with g:
-            g.set_tuning(1.0, pool)
+            g.set_tuning(1.0, pool, 512)
             compare_frames(g, r, 3)
         r.close()
 
@@ -645,6 +657,9 @@ def test_autotune_restores_state_and_keeps_results(gpu):
     with pytest.raises(Exception, match="stage_pool"):
         with Physics(ps) as c:
             c.set_tuning(1.0, 1000)
+    with pytest.raises(Exception, match="row_sort_threads"):
+        with Physics(ps) as c:
+            c.set_tuning(1.0, 896, 128)
 
 
 def test_checkpoint_rejects_other_sizes(gpu):
