@@ -278,6 +278,7 @@ def other_config(name: str, seed: int, steps: int) -> dict:
         phys.run(11)
         # the library's autotuner picks the launch choices (never the results)
         tuning, _ = phys.autotune(frames_per_trial=5)
+        graph = phys.frame_graph
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         phys.run(steps)
@@ -293,7 +294,7 @@ def other_config(name: str, seed: int, steps: int) -> dict:
         "spheres": n, "grids": cfg.grids, "spring": cfg.enable_spring, "layout": cfg.layout,
         "time_delta": cfg.time_delta, "steps": steps,
         "tuning": {"scan_margin": tuning[0], "stage_pool": tuning[1],
-                   "row_sort_threads": tuning[2]},
+                   "row_sort_threads": tuning[2], "frame_graph": graph},
         "value": n * steps / el, "unit": "sphere-steps/s", "ms_per_step": el * 1e3 / steps,
         "stage_ms": {"row_sort": float(stage_ms[0]), "collide_integrate": float(stage_ms[1])},
         "collide_roofline_frac": achieved / HBM_PEAK_GBS,
@@ -316,6 +317,8 @@ def main() -> int:
     ap.add_argument("--no-other-configs", action="store_true")
     ap.add_argument("--serving-trace", action="store_true",
                     help="only one batched drain of the serving mix (for a kernel trace)")
+    ap.add_argument("--no-frame-graph", action="store_true",
+                    help="launch every frame's kernels instead of replaying a captured graph")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a plain 64 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -340,6 +343,8 @@ def main() -> int:
     pos, vf = generate(cfg, seed=args.seed)
     torch.cuda.set_device(0)
     phys = Physics(cfg.settings())
+    if args.no_frame_graph:
+        phys.set_frame_graph(False)
     t0 = time.perf_counter()
     phys.upload_state(pos, vf)  # host -> HBM, synchronised by the library
     upload_ms = (time.perf_counter() - t0) * 1e3

@@ -204,6 +204,7 @@ struct FrameBuffers {
 struct FrameTuning {
     bool big_pool = false;     // collide stages kStagePoolBig slots
     bool narrow_sort = false;  // row sort blocks of 256 threads keeping kRowCapNarrow spheres in LDS
+    bool frame_graph = false;  // runs of LOOP frames replay a captured two-frame hipGraph
 };
 
 // Enqueues frame `time`: INIT at time 0 (offsets, copy, keys, buckets for

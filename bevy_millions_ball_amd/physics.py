@@ -236,7 +236,23 @@ class Physics:
         return float(t.scan_margin), int(t.stage_pool), int(t.row_sort_threads)
 
     def set_tuning(self, scan_margin: float, stage_pool: int, row_sort_threads: int = 512) -> None:
-        t = MbTuning(float(scan_margin), int(stage_pool), int(row_sort_threads))
+        t = MbTuning()
+        check(lib().mb_physics_get_tuning(self._h, C.byref(t)))
+        t.scan_margin, t.stage_pool, t.row_sort_threads = float(scan_margin), int(stage_pool), \
+            int(row_sort_threads)
+        check(lib().mb_physics_set_tuning(self._h, C.byref(t)))
+
+    @property
+    def frame_graph(self) -> bool:
+        """Whether runs of LOOP frames replay a captured two-frame hipGraph."""
+        t = MbTuning()
+        check(lib().mb_physics_get_tuning(self._h, C.byref(t)))
+        return bool(t.frame_graph)
+
+    def set_frame_graph(self, enable: bool) -> None:
+        t = MbTuning()
+        check(lib().mb_physics_get_tuning(self._h, C.byref(t)))
+        t.frame_graph = int(bool(enable))
         check(lib().mb_physics_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, frames_per_trial: int = 5) -> Tuple[Tuple[float, int, int], float]:

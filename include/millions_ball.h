@@ -139,12 +139,17 @@ int mb_physics_download_state(const mb_physics *physics, float *position_with_sc
  * 896 (7 blocks per CU) or 1536 (4 blocks per CU; fewer blocks of dense
  * scenes fall back to reading neighbours from global memory);
  * row_sort_threads is the width of a row sort block, 512 (rows of up to 4096
- * spheres sorted in LDS) or 256 (up to 2048; twice as many rows resident).
- * Defaults: 1.0, 896 and 512. */
+ * spheres sorted in LDS) or 256 (up to 2048; twice as many rows resident);
+ * frame_graph (0 / 1) lets mb_physics_step replay runs of LOOP frames as a
+ * captured hipGraph of two frames instead of launching each kernel (never
+ * while profiling; the frame after INIT and frames on the stage kernels are
+ * launched directly).  Defaults: 1.0, 896, 512 and 0.  The autotuner times
+ * its winner both ways and keeps the faster. */
 typedef struct mb_tuning {
     float scan_margin;
     uint32_t stage_pool;
     uint32_t row_sort_threads;
+    uint32_t frame_graph;
 } mb_tuning;
 
 int mb_physics_get_tuning(const mb_physics *physics, mb_tuning *tuning);

@@ -57,6 +57,7 @@ pub struct mb_tuning {
     pub scan_margin: f32,
     pub stage_pool: u32,
     pub row_sort_threads: u32,
+    pub frame_graph: u32,
 }
 
 #[repr(C)]
