@@ -317,8 +317,8 @@ def main() -> int:
     ap.add_argument("--no-other-configs", action="store_true")
     ap.add_argument("--serving-trace", action="store_true",
                     help="only one batched drain of the serving mix (for a kernel trace)")
-    ap.add_argument("--no-frame-graph", action="store_true",
-                    help="launch every frame's kernels instead of replaying a captured graph")
+    ap.add_argument("--frame-graph", action="store_true",
+                    help="replay runs of frames as a captured graph instead of launching each kernel")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a plain 64 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -343,8 +343,8 @@ def main() -> int:
     pos, vf = generate(cfg, seed=args.seed)
     torch.cuda.set_device(0)
     phys = Physics(cfg.settings())
-    if args.no_frame_graph:
-        phys.set_frame_graph(False)
+    if args.frame_graph:
+        phys.set_frame_graph(True)
     t0 = time.perf_counter()
     phys.upload_state(pos, vf)  # host -> HBM, synchronised by the library
     upload_ms = (time.perf_counter() - t0) * 1e3
