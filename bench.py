@@ -294,7 +294,7 @@ def other_config(name: str, seed: int, steps: int) -> dict:
         "spheres": n, "grids": cfg.grids, "spring": cfg.enable_spring, "layout": cfg.layout,
         "time_delta": cfg.time_delta, "steps": steps,
         "tuning": {"scan_margin": tuning[0], "stage_pool": tuning[1],
-                   "row_sort_threads": tuning[2], "frame_graph": graph},
+                   "row_sort_threads": tuning[2], "dense_scan": tuning[3], "frame_graph": graph},
         "value": n * steps / el, "unit": "sphere-steps/s", "ms_per_step": el * 1e3 / steps,
         "stage_ms": {"row_sort": float(stage_ms[0]), "collide_integrate": float(stage_ms[1])},
         "collide_roofline_frac": achieved / HBM_PEAK_GBS,
@@ -317,6 +317,8 @@ def main() -> int:
     ap.add_argument("--no-other-configs", action="store_true")
     ap.add_argument("--serving-trace", action="store_true",
                     help="only one batched drain of the serving mix (for a kernel trace)")
+    ap.add_argument("--tuning", default=None,
+                    help="scan_margin,stage_pool,row_sort_threads[,dense_scan] for the measured workload")
     ap.add_argument("--frame-graph", action="store_true",
                     help="replay runs of frames as a captured graph instead of launching each kernel")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -345,6 +347,9 @@ def main() -> int:
     phys = Physics(cfg.settings())
     if args.frame_graph:
         phys.set_frame_graph(True)
+    if args.tuning:
+        m, p, s, d = (args.tuning.split(",") + ["0"])[:4]
+        phys.set_tuning(float(m), int(p), int(s), int(d))
     t0 = time.perf_counter()
     phys.upload_state(pos, vf)  # host -> HBM, synchronised by the library
     upload_ms = (time.perf_counter() - t0) * 1e3

@@ -50,7 +50,8 @@ class MbBuffers(C.Structure):
 
 class MbTuning(C.Structure):
     _fields_ = [("scan_margin", C.c_float), ("stage_pool", C.c_uint32),
-                ("row_sort_threads", C.c_uint32), ("frame_graph", C.c_uint32)]
+                ("row_sort_threads", C.c_uint32), ("frame_graph", C.c_uint32),
+                ("dense_scan", C.c_uint32)]
 
 
 class MbServerConfig(C.Structure):

@@ -418,6 +418,9 @@ static int accept_state(const mb::PhysConsts &c, uint32_t n, const StateInfo &in
     if (info.bad)
         return fail(MB_ERR_INVALID_ARGUMENT, std::to_string(info.bad) +
                                                  " spheres with a non-finite position, radius or f16 half");
+    if (info.r_min < 0.0f)
+        return fail(MB_ERR_INVALID_ARGUMENT,
+                    "a negative radius (position_with_scale.w): radii must be >= 0");
     if (info.max_read >= n)
         return fail(MB_ERR_INVALID_ARGUMENT,
                     "a sphere lies so far past the map that its neighbour walk reads cell " +
@@ -876,6 +879,7 @@ int mb_physics_get_tuning(const mb_physics *p, mb_tuning *t)
     t->stage_pool = p->tuning.big_pool ? mb::kStagePoolBig : mb::kStagePool;
     t->row_sort_threads = p->tuning.narrow_sort ? 256u : mb::kSortThreads;
     t->frame_graph = p->tuning.frame_graph ? 1u : 0u;
+    t->dense_scan = p->tuning.dense_scan ? 1u : 0u;
     return MB_OK;
 }
 
@@ -889,12 +893,14 @@ int mb_physics_set_tuning(mb_physics *p, const mb_tuning *t)
     if (t->row_sort_threads != 256u && t->row_sort_threads != mb::kSortThreads)
         return fail(MB_ERR_INVALID_ARGUMENT, "row_sort_threads must be 256 or 512");
     if (t->frame_graph > 1u) return fail(MB_ERR_INVALID_ARGUMENT, "frame_graph must be 0 or 1");
+    if (t->dense_scan > 1u) return fail(MB_ERR_INVALID_ARGUMENT, "dense_scan must be 0 or 1");
     drop_graph(p);
     p->scan_margin = t->scan_margin;
     p->consts.scan_margin = t->scan_margin;
     p->tuning.big_pool = t->stage_pool == mb::kStagePoolBig;
     p->tuning.narrow_sort = t->row_sort_threads == 256u;
     p->tuning.frame_graph = t->frame_graph != 0u;
+    p->tuning.dense_scan = t->dense_scan != 0u;
     return MB_OK;
 }
 
@@ -914,16 +920,18 @@ int mb_physics_autotune(mb_physics *p, uint32_t frames, mb_tuning *best, double 
     const float margins[] = {1.0f, 0.5f, 0.25f};
     const uint32_t pools[] = {mb::kStagePool, mb::kStagePoolBig};
     const uint32_t sorts[] = {mb::kSortThreads, 256u};
+    const uint32_t dense[] = {0u, 1u};
     mb_tuning winner = keep;
     double winner_ms = 0.0;
     bool have = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MB_HIP(hipEventCreate(&e0));
     MB_HIP(hipEventCreate(&e1));
+    for (uint32_t ds : dense)
     for (uint32_t sort : sorts)
     for (uint32_t pool : pools) {
         for (float margin : margins) {
-            const mb_tuning t{margin, pool, sort, keep.frame_graph};
+            const mb_tuning t{margin, pool, sort, keep.frame_graph, ds};
             rc = mb_physics_load_checkpoint(p, ck.data(), ck.size());
             if (rc == MB_OK) rc = mb_physics_set_tuning(p, &t);
             if (rc == MB_OK && p->time < 0) rc = mb_physics_step(p, 1, nullptr);  // INIT

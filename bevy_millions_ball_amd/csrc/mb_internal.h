@@ -205,6 +205,7 @@ struct FrameTuning {
     bool big_pool = false;     // collide stages kStagePoolBig slots
     bool narrow_sort = false;  // row sort blocks of 256 threads keeping kRowCapNarrow spheres in LDS
     bool frame_graph = false;  // runs of LOOP frames replay a captured two-frame hipGraph
+    bool dense_scan = false;   // collide variant whose dense blocks take the split candidate scan
 };
 
 // Enqueues frame `time`: INIT at time 0 (offsets, copy, keys, buckets for

@@ -1292,9 +1292,14 @@ __device__ __forceinline__ Ranges make_ranges(uint32_t lo0, uint32_t hi0, uint32
     return r;
 }
 
-template <bool SPRING, bool FROM_LDS>
+// SPLIT (dense blocks, chosen per block): a chunk never crosses the end of a
+// range, so its staged candidates are consecutive in LDS and the scan steps a
+// pointer (13 VALU per candidate instead of 19).  Where ranges hold a few
+// candidates each (about one sphere per cell) the extra chunks cost more than
+// that saves, so the other form takes the three ranges in one chunk.
+template <bool SPRING, bool FROM_LDS, bool SPLIT>
 __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, uint32_t s,
-                                                const Ranges r,
+                                                const Ranges r, float rmax,
                                                 const float4 *__restrict__ s_pos,
                                                 const uint2 *__restrict__ s_vm,
                                                 const uint32_t *__restrict__ perm,
@@ -1316,27 +1321,49 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
     // exact test's 1.0001), which also covers the fused d2 of the scan
     const float own_reach_w = own_reach * 1.0001f;
     const uint32_t t_self = s - r.slot0;       // own slot, always in the middle range
-    // byte offset of candidate j's staged position
-    const uint32_t lb0 = r.lds0 * 16u, lb1 = r.lds1 * 16u, lb2 = r.lds2 * 16u;
     const char *s_pos_b = reinterpret_cast<const char *>(s_pos);
+    // byte offset of candidate j's staged position (the unsplit form)
+    const uint32_t lb0 = r.lds0 * 16u, lb1 = r.lds1 * 16u, lb2 = r.lds2 * 16u;
     uint32_t j = 0;
     while (j < total) {
-        const uint32_t end = min(j + 32u, total);
+        // split: candidate t of the chunk sits at LDS index t + lbase
+        const uint32_t bound = !SPLIT ? total : (j < r.n0 ? r.n0 : (j < r.n01 ? r.n01 : total));
+        const uint32_t end = min(j + 32u, bound);
+        const uint32_t lbase = r.lds0 + (j >= r.n0 ? r.lds1 : 0u) + (j >= r.n01 ? r.lds2 : 0u);
         const float rx = a.px, ry = a.py, rz = a.pz;
         // candidate t of the chunk sets bit end - 1 - t: the first one is the
         // highest bit, so a shift-and-add builds the mask
         uint32_t near = 0;
-        for (uint32_t t = j; t < end; ++t) {
-            float4 bp;
-            if (FROM_LDS)
-                bp = *reinterpret_cast<const float4 *>(
+        if (FROM_LDS && SPLIT) {
+            // one reach for the chunk, from the largest radius staged: never
+            // narrower than a candidate's own
+            const float reach = __builtin_fmaf(rmax, 1.0001f, own_reach_w);
+            const float reach2 = reach * reach;
+            const char *p = s_pos_b + (j + lbase) * 16u;
+            const char *pe = s_pos_b + (end + lbase) * 16u;
+            for (; p < pe; p += 16) {
+                const float4 bp = *reinterpret_cast<const float4 *>(p);
+                const float dx = rx - bp.x, dy = ry - bp.y, dz = rz - bp.z;
+                const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                near = near + near + (d2 <= reach2 ? 1u : 0u);
+            }
+        } else if (FROM_LDS) {
+            for (uint32_t t = j; t < end; ++t) {
+                const float4 bp = *reinterpret_cast<const float4 *>(
                     s_pos_b + (t * 16u + lb0 + (t >= r.n0 ? lb1 : 0u) + (t >= r.n01 ? lb2 : 0u)));
-            else
-                bp = pos_i[perm[slot_of(t)]];
-            const float reach = __builtin_fmaf(bp.w, 1.0001f, own_reach_w);
-            const float dx = rx - bp.x, dy = ry - bp.y, dz = rz - bp.z;
-            const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-            near = near + near + (d2 <= reach * reach ? 1u : 0u);
+                const float reach = __builtin_fmaf(bp.w, 1.0001f, own_reach_w);
+                const float dx = rx - bp.x, dy = ry - bp.y, dz = rz - bp.z;
+                const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                near = near + near + (d2 <= reach * reach ? 1u : 0u);
+            }
+        } else {
+            for (uint32_t t = j; t < end; ++t) {
+                const float4 bp = pos_i[perm[slot_of(t)]];
+                const float reach = __builtin_fmaf(bp.w, 1.0001f, own_reach_w);
+                const float dx = rx - bp.x, dy = ry - bp.y, dz = rz - bp.z;
+                const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                near = near + near + (d2 <= reach * reach ? 1u : 0u);
+            }
         }
         if (t_self >= j && t_self < end) near &= ~(1u << (end - 1u - t_self));
         uint32_t next = end;
@@ -1346,7 +1373,7 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
             near ^= 0x80000000u >> lz;
             bool touch;
             if (FROM_LDS) {
-                const uint32_t l = lds_of(t);
+                const uint32_t l = SPLIT ? t + lbase : lds_of(t);
                 touch = collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
             } else {
                 const uint32_t bo = perm[slot_of(t)];
@@ -1372,7 +1399,10 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
 // ranges (physics.wgsl:119-206) in the shader's order.  The block's first and
 // last cells come from the sorted keys, so the window bounds are uniform
 // loads issued at launch and one barrier separates staging from the walk.
-template <bool SPRING, uint32_t POOL, bool PRIO = false>
+// DENSE: blocks whose slots hold 2 or more spheres per cell take the split
+// candidate scan (a kernel variant of its own, so that scenes near one sphere
+// per cell run the code without it).
+template <bool SPRING, uint32_t POOL, bool PRIO = false, bool DENSE = false>
 __device__ __forceinline__ void physics_step_body(
     const PhysConsts &c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
     uint32_t *__restrict__ keys, const float4 *__restrict__ pos_i,
@@ -1434,16 +1464,27 @@ __device__ __forceinline__ void physics_step_body(
     const uint32_t wbase[3] = {0u, wcnt[0], wcnt[0] + wcnt[1]};
     const uint32_t wtot = wcnt[0] + wcnt[1] + wcnt[2];
     const bool staged = wtot <= POOL;
+    // dense block: its slots span few cells (2 or more spheres per cell)
+    const bool split = DENSE && staged && (klast - kfirst + 1u) * 2u <= last + 1u;
+    __shared__ float s_wave_rmax[4];
     if (staged) {
         // one pass over the pool: index j of window w holds slot
         // wlo[w] + j - wbase[w] (u32 arithmetic, wrapping)
         const uint32_t d1 = (wlo[1] - wbase[1]) - wlo[0];
         const uint32_t d2 = (wlo[2] - wbase[2]) - (wlo[1] - wbase[1]);
+        float rmax = -INFINITY;
         for (uint32_t j = tid; j < wtot; j += 256u) {
             const uint32_t bo =
                 perm[j + wlo[0] + (j >= wbase[1] ? d1 : 0u) + (j >= wbase[2] ? d2 : 0u)];
-            s_pos[j] = pos_i[bo];
+            const float4 bp = pos_i[bo];
+            s_pos[j] = bp;
+            if (DENSE) rmax = fmaxf(rmax, bp.w);
             if (!SPRING) s_vm[j] = vel_mass_of(vfme_i[bo]);
+        }
+        if (split) {
+            // the largest radius in the windows, for the dense scan's reach
+            for (uint32_t o = 32; o > 0; o >>= 1) rmax = fmaxf(rmax, __shfl_xor(rmax, o, 64));
+            if ((tid & 63u) == 0u) s_wave_rmax[tid >> 6] = rmax;
         }
     }
     __syncthreads();
@@ -1454,10 +1495,15 @@ __device__ __forceinline__ void physics_step_body(
                                   wbase[2]);
     verlet(c, a);
     const float ox = a.px, oy = a.py, oz = a.pz;
-    if (staged)
-        walk_candidates<SPRING, true>(c, a, s, rg, s_pos, s_vm, perm, pos_i, vfme_i);
-    else
-        walk_candidates<SPRING, false>(c, a, s, rg, s_pos, s_vm, perm, pos_i, vfme_i);
+    if (DENSE && split) {
+        const float rmax = fmaxf(fmaxf(s_wave_rmax[0], s_wave_rmax[1]),
+                                 fmaxf(s_wave_rmax[2], s_wave_rmax[3]));
+        walk_candidates<SPRING, true, true>(c, a, s, rg, rmax, s_pos, s_vm, perm, pos_i, vfme_i);
+    } else if (staged) {
+        walk_candidates<SPRING, true, false>(c, a, s, rg, 0.0f, s_pos, s_vm, perm, pos_i, vfme_i);
+    } else {
+        walk_candidates<SPRING, false, false>(c, a, s, rg, 0.0f, s_pos, s_vm, perm, pos_i, vfme_i);
+    }
     jitter(a, s, ox, oy, oz);
     const uint32_t next = finish_agent<SPRING, true>(c, a, s, pos_o, vfme_o, keys);
     const uint32_t nz = fast_div(next, c.grids_div);
@@ -1466,7 +1512,7 @@ __device__ __forceinline__ void physics_step_body(
                next - nz * G, s0, last);
 }
 
-template <bool SPRING, uint32_t POOL, bool PRIO>
+template <bool SPRING, uint32_t POOL, bool PRIO, bool DENSE>
 __global__ __launch_bounds__(256) void physics_step_kernel(
     PhysConsts c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
     uint32_t *__restrict__ keys, const float4 *__restrict__ pos_i,
@@ -1475,9 +1521,9 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
 {
     __shared__ float4 s_pos[POOL];  // the three windows back to back
     __shared__ uint2 s_vm[POOL];    // their velocity and mass halves
-    physics_step_body<SPRING, POOL, PRIO>(c, perm, off, keys, pos_i, vfme_i, pos_o, vfme_o, rb,
-                                          set_out,
-                                    n, xcd_block(blockIdx.x, gridDim.x), s_pos, s_vm);
+    physics_step_body<SPRING, POOL, PRIO, DENSE>(c, perm, off, keys, pos_i, vfme_i, pos_o, vfme_o,
+                                                 rb, set_out, n, xcd_block(blockIdx.x, gridDim.x),
+                                                 s_pos, s_vm);
 }
 
 // INIT frame of the frame path: the sort of the all-zero keys is the
@@ -1555,29 +1601,35 @@ __global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *
     bucket_file<kSpread>(rb, rb.spread, rb.spill_count + set * kFillStride, cz, cx, s);
 }
 
-template <uint32_t POOL, bool PRIO>
-static void launch_collide_prio(const FrameBuffers &b, const PhysConsts &c, uint32_t set_out,
-                                uint32_t n, hipStream_t stream)
+template <uint32_t POOL, bool PRIO, bool DENSE>
+static void launch_collide_variant(const FrameBuffers &b, const PhysConsts &c, uint32_t set_out,
+                                   uint32_t n, hipStream_t stream)
 {
     const dim3 grid((n + 255u) / 256u);
     if (c.enable_spring)
-        hipLaunchKernelGGL((physics_step_kernel<true, POOL, PRIO>), grid, dim3(256), 0, stream, c,
-                           b.perm, b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, b.rb,
-                           set_out, n);
+        hipLaunchKernelGGL((physics_step_kernel<true, POOL, PRIO, DENSE>), grid, dim3(256), 0,
+                           stream, c, b.perm, b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o,
+                           b.vfme_o, b.rb, set_out, n);
     else
-        hipLaunchKernelGGL((physics_step_kernel<false, POOL, PRIO>), grid, dim3(256), 0, stream, c,
-                           b.perm, b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, b.rb,
-                           set_out, n);
+        hipLaunchKernelGGL((physics_step_kernel<false, POOL, PRIO, DENSE>), grid, dim3(256), 0,
+                           stream, c, b.perm, b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o,
+                           b.vfme_o, b.rb, set_out, n);
 }
 
 template <uint32_t POOL>
 static void launch_collide(const FrameBuffers &b, const PhysConsts &c, uint32_t set_out,
-                           uint32_t n, hipStream_t stream)
+                           uint32_t n, hipStream_t stream, bool dense)
 {
-    if (c.gather_priority)
-        launch_collide_prio<POOL, true>(b, c, set_out, n, stream);
-    else
-        launch_collide_prio<POOL, false>(b, c, set_out, n, stream);
+    if (c.gather_priority) {
+        if (dense)
+            launch_collide_variant<POOL, true, true>(b, c, set_out, n, stream);
+        else
+            launch_collide_variant<POOL, true, false>(b, c, set_out, n, stream);
+    } else if (dense) {
+        launch_collide_variant<POOL, false, true>(b, c, set_out, n, stream);
+    } else {
+        launch_collide_variant<POOL, false, false>(b, c, set_out, n, stream);
+    }
 }
 
 hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time, uint32_t n,
@@ -1613,9 +1665,9 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time
                            b.scratch_idx, G, n, b.perm, b.off, b.keys_out);
     if ((e = mark(1)) != hipSuccess) return e;
     if (tuning.big_pool)
-        launch_collide<kStagePoolBig>(b, c, set ^ 1u, n, stream);
+        launch_collide<kStagePoolBig>(b, c, set ^ 1u, n, stream, tuning.dense_scan);
     else
-        launch_collide<kStagePool>(b, c, set ^ 1u, n, stream);
+        launch_collide<kStagePool>(b, c, set ^ 1u, n, stream, tuning.dense_scan);
     if ((e = mark(2)) != hipSuccess) return e;
     return hipGetLastError();
 }

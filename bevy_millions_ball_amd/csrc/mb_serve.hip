@@ -732,6 +732,9 @@ int mb_server_submit(mb_server *s, const mb_settings *settings, const float *pos
             r_min = std::min(r_min, q[3]);
             r_max = std::max(r_max, q[3]);
         }
+        if (r_min < 0.0f)
+            return fail(MB_ERR_INVALID_ARGUMENT,
+                        "a negative radius (position_with_scale.w): radii must be >= 0");
         if (mb::loop_cell_bound(c, r_min, r_max) >= g)
             return fail(MB_ERR_INVALID_ARGUMENT,
                         "the radii let wall-clamped positions leave the grid (serving needs every cell in it)");

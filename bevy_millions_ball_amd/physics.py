@@ -229,17 +229,18 @@ class Physics:
 
     # -- tuning (launch choices that do not change results) ---------------
     @property
-    def tuning(self) -> Tuple[float, int, int]:
-        """(scan_margin, stage_pool, row_sort_threads)."""
+    def tuning(self) -> Tuple[float, int, int, int]:
+        """(scan_margin, stage_pool, row_sort_threads, dense_scan)."""
         t = MbTuning()
         check(lib().mb_physics_get_tuning(self._h, C.byref(t)))
-        return float(t.scan_margin), int(t.stage_pool), int(t.row_sort_threads)
+        return float(t.scan_margin), int(t.stage_pool), int(t.row_sort_threads), int(t.dense_scan)
 
-    def set_tuning(self, scan_margin: float, stage_pool: int, row_sort_threads: int = 512) -> None:
+    def set_tuning(self, scan_margin: float, stage_pool: int, row_sort_threads: int = 512,
+                   dense_scan: int = 0) -> None:
         t = MbTuning()
         check(lib().mb_physics_get_tuning(self._h, C.byref(t)))
-        t.scan_margin, t.stage_pool, t.row_sort_threads = float(scan_margin), int(stage_pool), \
-            int(row_sort_threads)
+        t.scan_margin, t.stage_pool = float(scan_margin), int(stage_pool)
+        t.row_sort_threads, t.dense_scan = int(row_sort_threads), int(dense_scan)
         check(lib().mb_physics_set_tuning(self._h, C.byref(t)))
 
     @property
@@ -255,14 +256,15 @@ class Physics:
         t.frame_graph = int(bool(enable))
         check(lib().mb_physics_set_tuning(self._h, C.byref(t)))
 
-    def autotune(self, frames_per_trial: int = 5) -> Tuple[Tuple[float, int, int], float]:
+    def autotune(self, frames_per_trial: int = 5) -> Tuple[Tuple[float, int, int, int], float]:
         """Times every candidate tuning on a copy of the state, keeps the
-        fastest; returns ((scan_margin, stage_pool, row_sort_threads), ms per
-        frame)."""
+        fastest; returns ((scan_margin, stage_pool, row_sort_threads,
+        dense_scan), ms per frame)."""
         t = MbTuning()
         ms = C.c_double()
         check(lib().mb_physics_autotune(self._h, int(frames_per_trial), C.byref(t), C.byref(ms)))
-        return (float(t.scan_margin), int(t.stage_pool), int(t.row_sort_threads)), ms.value
+        return (float(t.scan_margin), int(t.stage_pool), int(t.row_sort_threads),
+                int(t.dense_scan)), ms.value
 
     # -- checkpoints --------------------------------------------------------
     def save_checkpoint(self) -> bytes:

@@ -115,7 +115,8 @@ int mb_physics_get_settings(const mb_physics *physics, mb_settings *settings);
  * time to -1 so that the next frame runs the INIT pipeline.  Cells are the
  * shader's, unclamped (physics.wgsl:107-117): a sphere past the map's +x / +z
  * edge has a key that aliases a cell of a later row, as in the reference.
- * Rejects non-finite positions, radii or velocities, and a state for which
+ * Rejects non-finite positions, radii or velocities, negative radii (the
+ * candidate filter of the collide kernels assumes r >= 0), and a state for which
  * the reference would read its N-entry start / close arrays out of bounds: a
  * sphere whose neighbour walk reaches cell key N, or radii that let a
  * wall-clamped position reach one later.  After a rejected upload the handle
@@ -143,13 +144,19 @@ int mb_physics_download_state(const mb_physics *physics, float *position_with_sc
  * frame_graph (0 / 1) lets mb_physics_step replay runs of LOOP frames as a
  * captured hipGraph of two frames instead of launching each kernel (never
  * while profiling; the frame after INIT and frames on the stage kernels are
- * launched directly).  Defaults: 1.0, 896, 512 and 0.  The autotuner times
- * its winner both ways and keeps the faster. */
+ * launched directly); dense_scan (0 / 1) runs the collide variant in which
+ * blocks holding 2 or more spheres per cell scan their candidates range by
+ * range with one reach from the block's largest radius (fewer instructions
+ * per candidate where cells are crowded, a little slower near one sphere per
+ * cell).  Defaults: 1.0, 896, 512, 0 and 0.  The autotuner searches
+ * scan_margin x stage_pool x row_sort_threads x dense_scan, then times its
+ * winner with the other frame_graph choice. */
 typedef struct mb_tuning {
     float scan_margin;
     uint32_t stage_pool;
     uint32_t row_sort_threads;
     uint32_t frame_graph;
+    uint32_t dense_scan;
 } mb_tuning;
 
 int mb_physics_get_tuning(const mb_physics *physics, mb_tuning *tuning);

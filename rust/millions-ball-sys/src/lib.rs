@@ -58,6 +58,7 @@ pub struct mb_tuning {
     pub stage_pool: u32,
     pub row_sort_threads: u32,
     pub frame_graph: u32,
+    pub dense_scan: u32,
 }
 
 #[repr(C)]

@@ -360,9 +360,15 @@ def test_upload_rejects_walks_past_the_arrays(gpu):
             g.upload_state(pos, vf)
         with pytest.raises(Exception, match="no accepted state"):
             g.run(1)
-        # an in-map state is accepted afterwards
+        # negative radii are refused too
         cfg = replace(CONFIGS["plain_65k"], number_of_agents=n, grids=64)
-        g.upload_state(*generate(cfg, seed=1))
+        good, gvf = generate(cfg, seed=1)
+        bad = good.copy()
+        bad[9, 3] = -0.5
+        with pytest.raises(Exception, match="negative radius"):
+            g.upload_state(bad, gvf)
+        # an in-map state is accepted afterwards
+        g.upload_state(good, gvf)
         g.run(2)
 
 
@@ -610,13 +616,14 @@ def test_checkpoint_resume_is_bit_exact(gpu, split):
         assert np.array_equal(res[1], want[1])
 
 
-TUNINGS = [(m, p, s) for s in (512, 256) for p in (896, 1536) for m in (1.0, 0.5, 0.25)]
+TUNINGS = [(m, p, s, d) for d in (0, 1) for s in (512, 256) for p in (896, 1536)
+           for m in (1.0, 0.5, 0.25)]
 
 
 @pytest.mark.parametrize("scene", ["uniform", "dense", "clumps"])
 def test_every_tuning_matches_the_oracle(gpu, oracle, scene):
     """Every point the autotuner can pick (scan margin x staging pool x row
-    sort width, the spring-clumps winner 0.25 / 1536 among them) runs the
+    sort width x dense-scan variant, the spring-clumps winner among them) runs the
     oracle's frames bit for bit on a uniform scene, a scene with dense rows
     (bucket spill, rows sorted in global memory, unstaged collide blocks) and
     a spring-clumps scene."""
@@ -638,21 +645,22 @@ def test_every_tuning_matches_the_oracle(gpu, oracle, scene):
     pr, vr = r.download()
     kr = r.debug()["next_keys"]
     r.close()
-    for margin, pool, sort in TUNINGS:
-        print(f"[tuning] {scene} {margin} {pool} {sort}", flush=True)
+    for tuning in TUNINGS:
+        print(f"[tuning] {scene} {tuning}", flush=True)
         with Physics(ps) as g:
-            g.set_tuning(margin, pool, sort)
-            assert g.tuning == (margin, pool, sort)
+            g.set_tuning(*tuning)
+            assert g.tuning == tuning
             g.upload_state(pos, vf)
             g.run(6)
             pg, vg = g.download_state()
-            assert np.array_equal(pg.view(np.uint32), pr.view(np.uint32)), (margin, pool, sort)
-            assert np.array_equal(vg, vr), (margin, pool, sort)
-            assert np.array_equal(g.debug()["next_keys"], kr), (margin, pool, sort)
+            assert np.array_equal(pg.view(np.uint32), pr.view(np.uint32)), tuning
+            assert np.array_equal(vg, vr), tuning
+            assert np.array_equal(g.debug()["next_keys"], kr), tuning
 
 
+@pytest.mark.parametrize("dense", [0, 1])
 @pytest.mark.parametrize("pool", [896, 1536])
-def test_large_scene_collide_variant_bit_exact(gpu, oracle, pool):
+def test_large_scene_collide_variant_bit_exact(gpu, oracle, pool, dense):
     """Scenes above 2^21 spheres launch the collide variant that raises a
     block's wave priority while it stages its windows; its spring and plain
     instantiations, with either pool, must give the oracle's frames."""
@@ -662,7 +670,7 @@ def test_large_scene_collide_variant_bit_exact(gpu, oracle, pool):
         print(f"[large] {name} n={n} pool={pool}", flush=True)
         g, r = make_pair(oracle, cfg, seed=13)
         with g:
-            g.set_tuning(1.0, pool, 512)
+            g.set_tuning(1.0, pool, 512, dense)
             compare_frames(g, r, 3)
         r.close()
 
