@@ -317,6 +317,8 @@ def main() -> int:
     ap.add_argument("--no-other-configs", action="store_true")
     ap.add_argument("--serving-trace", action="store_true",
                     help="only one batched drain of the serving mix (for a kernel trace)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let the library's autotuner pick the launch choices before timing")
     ap.add_argument("--tuning", default=None,
                     help="scan_margin,stage_pool,row_sort_threads[,dense_scan] for the measured workload")
     ap.add_argument("--frame-graph", action="store_true",
@@ -355,6 +357,12 @@ def main() -> int:
     upload_ms = (time.perf_counter() - t0) * 1e3
     phys.run(1)  # INIT frame
     phys.run(args.warmup)
+    tuning = None
+    if args.autotune:
+        # the library times its candidate launch choices on a copy of the
+        # state and keeps the fastest (results are the same for all of them)
+        tuning, _ = phys.autotune(frames_per_trial=10)
+        tuning = tuning + (phys.frame_graph,)
     torch.cuda.synchronize()
 
     # -- throughput: K frames back to back -----------------------------------
@@ -419,6 +427,9 @@ def main() -> int:
             "layout": cfg.layout,
             "seed": args.seed,
             "parallelism": "single GPU",
+            "tuning": None if tuning is None else {
+                "scan_margin": tuning[0], "stage_pool": tuning[1], "row_sort_threads": tuning[2],
+                "dense_scan": tuning[3], "frame_graph": tuning[4]},
         },
         "p50_step_ms": float(np.percentile(lat, 50)),
         "p99_step_ms": float(np.percentile(lat, 99)),

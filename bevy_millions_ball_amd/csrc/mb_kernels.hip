@@ -1392,6 +1392,86 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
     }
 }
 
+// The dense blocks' walk (the SPLIT form from LDS, one reach from the block's
+// largest radius), with the exact pass run over groups of up to four chunks
+// at a time: a wave then iterates as often as its busiest lane has near
+// candidates in the group, where chunk by chunk it iterates as often as the
+// sum of each chunk's busiest lane.  Each chunk lies inside one range, so its
+// staged candidates are consecutive (candidate t at LDS index t + lb[g]).
+// The order of the exact tests is the same: chunk by chunk, candidate by
+// candidate; a contact that moves the sphere beyond the margin still restarts
+// the scan right after that candidate.
+constexpr int kDenseGroup = 4;
+
+template <bool SPRING>
+__device__ __forceinline__ void walk_dense(const PhysConsts &c, Agent &a, uint32_t s,
+                                           const Ranges r, float rmax,
+                                           const float4 *__restrict__ s_pos,
+                                           const uint2 *__restrict__ s_vm)
+{
+    const uint32_t total = r.total;
+    const float margin = c.scan_margin * a.scale;
+    const float still = (margin * 0.999f) * (margin * 0.999f);
+    // the scan's reach, as in walk_candidates, from the largest staged radius
+    const float own_reach_w = (a.scale + margin) * 1.0001f;
+    const float reach = __builtin_fmaf(rmax, 1.0001f, own_reach_w);
+    const float reach2 = reach * reach;
+    const uint32_t t_self = s - r.slot0;
+    const char *s_pos_b = reinterpret_cast<const char *>(s_pos);
+    uint32_t j = 0;
+    while (j < total) {
+        const float rx = a.px, ry = a.py, rz = a.pz;
+        uint32_t mask[kDenseGroup], end[kDenseGroup], lb[kDenseGroup];
+        uint32_t b = j;
+#pragma unroll
+        for (int g = 0; g < kDenseGroup; ++g) {
+            const uint32_t bound = b < r.n0 ? r.n0 : (b < r.n01 ? r.n01 : total);
+            const uint32_t e = min(b + 32u, bound);  // e == b once b reaches total
+            lb[g] = r.lds0 + (b >= r.n0 ? r.lds1 : 0u) + (b >= r.n01 ? r.lds2 : 0u);
+            uint32_t m = 0;
+            const char *p = s_pos_b + (b + lb[g]) * 16u;
+            const char *pe = s_pos_b + (e + lb[g]) * 16u;
+            for (; p < pe; p += 16) {
+                const float4 bp = *reinterpret_cast<const float4 *>(p);
+                const float dx = rx - bp.x, dy = ry - bp.y, dz = rz - bp.z;
+                const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                m = m + m + (d2 <= reach2 ? 1u : 0u);
+            }
+            // candidate t of chunk g sets bit e - 1 - t
+            if (t_self >= b && t_self < e) m &= ~(1u << (e - 1u - t_self));
+            mask[g] = m;
+            end[g] = e;
+            b = e;
+        }
+        uint32_t next = b;
+        for (;;) {
+            // the first chunk with a near candidate left, chosen without a branch
+            const bool h0 = mask[0] != 0u, h1 = mask[1] != 0u, h2 = mask[2] != 0u;
+            const uint32_t m = h0 ? mask[0] : (h1 ? mask[1] : (h2 ? mask[2] : mask[3]));
+            if (m == 0u) break;
+            const uint32_t e = h0 ? end[0] : (h1 ? end[1] : (h2 ? end[2] : end[3]));
+            const uint32_t l0 = h0 ? lb[0] : (h1 ? lb[1] : (h2 ? lb[2] : lb[3]));
+            const uint32_t lz = (uint32_t)__builtin_clz(m);
+            const uint32_t t = e - 32u + lz;
+            const uint32_t bit = 0x80000000u >> lz;
+            mask[0] ^= h0 ? bit : 0u;
+            mask[1] ^= (!h0 && h1) ? bit : 0u;
+            mask[2] ^= (!h0 && !h1 && h2) ? bit : 0u;
+            mask[3] ^= (!h0 && !h1 && !h2) ? bit : 0u;
+            const uint32_t l = t + l0;
+            const bool touch = collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
+            if (touch) {  // only a contact moves the sphere
+                const float ex = a.px - rx, ey = a.py - ry, ez = a.pz - rz;
+                if (!(ex * ex + ey * ey + ez * ez <= still)) {
+                    next = t + 1u;  // moved beyond the margin: rescan after t
+                    break;
+                }
+            }
+        }
+        j = next;
+    }
+}
+
 // main() with LOOP_PIPELINE for a block of 256 consecutive sorted slots.
 // The slots' cells are non-decreasing, so the candidates of the whole block
 // in each of the three rows form one contiguous slot window; the windows are
@@ -1498,7 +1578,7 @@ __device__ __forceinline__ void physics_step_body(
     if (DENSE && split) {
         const float rmax = fmaxf(fmaxf(s_wave_rmax[0], s_wave_rmax[1]),
                                  fmaxf(s_wave_rmax[2], s_wave_rmax[3]));
-        walk_candidates<SPRING, true, true>(c, a, s, rg, rmax, s_pos, s_vm, perm, pos_i, vfme_i);
+        walk_dense<SPRING>(c, a, s, rg, rmax, s_pos, s_vm);
     } else if (staged) {
         walk_candidates<SPRING, true, false>(c, a, s, rg, 0.0f, s_pos, s_vm, perm, pos_i, vfme_i);
     } else {

@@ -237,3 +237,33 @@ def test_submit_rejects_states_outside_the_grid(gpu):
         rid = srv.submit(st, pos, vf, 2)
         srv.drain()
         srv.collect(rid)
+
+
+def test_bench_request_mix_subset(gpu, oracle):
+    """The first 10 requests of bench.py's own serving batch (poisson_requests
+    with its seed 0: N log-uniform in [4096, 1,048,576], G = next pow2 of
+    ceil(sqrt N), 1-60 LOOP steps): each request's result in the mix equals
+    its result alone, bit for bit, and the oracle's."""
+    from bevy_millions_ball_amd.serving import poisson_requests, request_scene
+
+    reqs = poisson_requests(64, rate_per_s=200.0, seed=0)[:10]
+    items = []
+    for r in reqs:
+        st, pos, vf = request_scene(r)
+        items.append((st, pos, vf, r.steps))
+    print("[bench mix] N =", [r.number_of_agents for r in reqs], "steps =", [r.steps for r in reqs],
+          flush=True)
+    alone = run_alone(items)
+    mixed, stats = run_mix(items)
+    for i, (a, b) in enumerate(zip(alone, mixed)):
+        assert_same(a, b, f"request {i} (N={reqs[i].number_of_agents})")
+    assert stats["sphere_steps"] == sum(r.number_of_agents * r.steps for r in reqs)
+    for (st, pos, vf, steps), got in zip(items, mixed):
+        w = oracle.World(st)
+        w.upload(pos, vf)
+        w.step(1 + steps)
+        p2, v2 = w.download()
+        assert np.array_equal(w.debug()["next_keys"], got.next_keys)
+        w.close()
+        assert np.array_equal(p2.view(np.uint32), got.position_with_scale.view(np.uint32))
+        assert np.array_equal(v2, got.packed_vfme)
