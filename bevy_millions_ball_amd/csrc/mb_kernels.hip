@@ -1160,6 +1160,7 @@ __device__ __forceinline__ void row_sort_body(
         __syncthreads();
         // s_cell[c] is now the end of cell c inside the row; rank each index
         // among its cell's indices and write its slot's perm and key there.
+        // Crowded cells are compared four indices per 16-byte LDS read.
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) {
             const uint32_t e = j * NT + tid;
@@ -1169,7 +1170,16 @@ __device__ __forceinline__ void row_sort_body(
                 const uint32_t cs = c ? s_cell[c - 1u] : 0u;
                 const uint32_t ce = s_cell[c];
                 uint32_t rank = 0;
-                for (uint32_t q = cs; q < ce; ++q) rank += s_idx[q] < idx;
+                uint32_t q = cs;
+                if (ce - cs > 8u) {
+                    for (; q & 3u; ++q) rank += s_idx[q] < idx;
+                    for (; q + 4u <= ce; q += 4u) {
+                        const uint4 w = *reinterpret_cast<const uint4 *>(s_idx + q);
+                        rank += (w.x < idx ? 1u : 0u) + (w.y < idx ? 1u : 0u) +
+                                (w.z < idx ? 1u : 0u) + (w.w < idx ? 1u : 0u);
+                    }
+                }
+                for (; q < ce; ++q) rank += s_idx[q] < idx;
                 perm[base + cs + rank] = idx;
                 sorted_keys[base + cs + rank] = r * grids + c;
             }
