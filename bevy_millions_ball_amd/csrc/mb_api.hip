@@ -242,7 +242,7 @@ struct mb_physics {
     bool last_stage = false;     // the last frame ran on the stage kernels
     float r_min = 0.0f, r_max = 0.0f;  // radii of the state (frames copy them)
     void *sort_ws = nullptr;     // stage-kernel sort workspace (allocated on first need)
-    mb::StateCheck *check = nullptr;   // device result of launch_state_check
+    mb::StateCheck *check = nullptr;   // launch_state_check's result [0] and partials
     hipStream_t last_stream = nullptr; // stream of the last stream-ordered call
 
     // Runs of frame-path LOOP frames replay a captured hipGraph of two frames
@@ -465,7 +465,7 @@ int mb_physics_create(const mb_settings *settings, mb_physics **out)
     ok = ok && hipMalloc(&p->spill, n * 8u) == hipSuccess;
     ok = ok && hipMalloc(&p->scratch_pairs, n * 8u) == hipSuccess;
     ok = ok && hipMalloc(&p->scratch_idx, n * 4u) == hipSuccess;
-    ok = ok && hipMalloc(&p->check, sizeof(mb::StateCheck)) == hipSuccess;
+    ok = ok && hipMalloc(&p->check, (1 + mb::kStateCheckParts) * sizeof(mb::StateCheck)) == hipSuccess;
     if (!ok) {
         release(p);
         delete p;

@@ -108,10 +108,8 @@ hipError_t launch_physics(const PhysConsts &c, bool init, const uint32_t *perm,
                           uint32_t *key_out, const float4 *pos_i, const uint4 *vfme_i,
                           float4 *pos_o, uint4 *vfme_o, uint32_t n, hipStream_t stream);
 
-// Result of launch_state_check; the host sets max_* and bad to 0, r_min_key
-// to 0xFFFFFFFF and r_max_key to 0 before the launch.  Radii are kept as
-// order-preserving u32 keys (sign bit flipped for positives, all bits for
-// negatives).
+// Result of launch_state_check.  Radii are kept as order-preserving u32 keys
+// (sign bit flipped for positives, all bits for negatives).
 struct StateCheck {
     unsigned long long max_read;  // largest start / close index the LOOP walk reads
     unsigned long long max_key;   // largest cell key (no u32 wrapping)
@@ -121,6 +119,10 @@ struct StateCheck {
     uint32_t pad;
 };
 
+// `out` holds 1 + kStateCheckParts entries: out[0] receives the result of
+// the N > 0 spheres, the others the per-block partials.  N = 0 launches
+// nothing (out[0] keeps what the host put there).
+constexpr uint32_t kStateCheckParts = 1024;
 hipError_t launch_state_check(const PhysConsts &c, const float4 *pos, const uint4 *vfme,
                               uint32_t *key_out, uint32_t n, StateCheck *out, hipStream_t stream);
 

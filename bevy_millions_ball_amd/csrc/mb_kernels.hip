@@ -1928,57 +1928,88 @@ __device__ __forceinline__ bool finite_word_halves(uint32_t w)
     return ((w >> 10) & 0x1Fu) != 0x1Fu && ((w >> 26) & 0x1Fu) != 0x1Fu;
 }
 
+__device__ __forceinline__ void combine_check(StateCheck &a, const StateCheck &b)
+{
+    a.max_read = max(a.max_read, b.max_read);
+    a.max_key = max(a.max_key, b.max_key);
+    a.r_min_key = min(a.r_min_key, b.r_min_key);
+    a.r_max_key = max(a.r_max_key, b.r_max_key);
+    a.bad += b.bad;
+}
+
+// Folds the 256 per-thread results of a block into thread 0's `v` (waves by
+// shuffles, the four wave results through LDS).
+__device__ __forceinline__ void block_reduce_check(StateCheck &v)
+{
+    __shared__ StateCheck s_part[4];
+    for (uint32_t off = 32; off > 0; off >>= 1) {
+        StateCheck o;
+        o.max_read = __shfl_xor(v.max_read, off, 64);
+        o.max_key = __shfl_xor(v.max_key, off, 64);
+        o.r_min_key = __shfl_xor(v.r_min_key, off, 64);
+        o.r_max_key = __shfl_xor(v.r_max_key, off, 64);
+        o.bad = __shfl_xor(v.bad, off, 64);
+        combine_check(v, o);
+    }
+    if ((threadIdx.x & 63u) == 0u) s_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0u)
+        for (uint32_t w = 1; w < 4u; ++w) combine_check(v, s_part[w]);
+}
+
+// One partial result per block in part[blockIdx.x]: a reduction with no
+// atomics (one address taking an atomic from every wave serialises them).
 __global__ __launch_bounds__(256) void state_check_kernel(PhysConsts c,
                                                           const float4 *__restrict__ pos,
                                                           const uint4 *__restrict__ vfme,
                                                           uint32_t *__restrict__ key_out,
-                                                          uint32_t n, StateCheck *__restrict__ out)
+                                                          uint32_t n, StateCheck *__restrict__ part)
 {
     const uint64_t G = c.grids;
-    unsigned long long max_read = 0, max_key = 0;
-    uint32_t rmin = 0xFFFFFFFFu, rmax = 0u, bad = 0u;
+    StateCheck v{0ull, 0ull, 0xFFFFFFFFu, 0u, 0u, 0u};
     for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
         const float4 p = pos[i];
         const uint4 q = vfme[i];
         const bool fin = isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(p.w) &&
                          finite_word_halves(q.x) && finite_word_halves(q.y) &&
                          finite_word_halves(q.z) && finite_word_halves(q.w);
-        bad += fin ? 0u : 1u;
+        v.bad += fin ? 0u : 1u;
         uint32_t cx, cz;
         cell_of(c, p.x, p.z, cx, cz);
         const uint64_t k = (uint64_t)cz * G + cx;
         const uint64_t right = (uint64_t)cx + 1u < G ? 1u : 0u;
         const uint64_t rd = (uint64_t)cz + 1u < G ? k + G + right : k + right;
-        max_key = max(max_key, (unsigned long long)k);
-        max_read = max(max_read, (unsigned long long)rd);
+        v.max_key = max(v.max_key, (unsigned long long)k);
+        v.max_read = max(v.max_read, (unsigned long long)rd);
         const uint32_t ok = float_order_key(p.w);
-        rmin = min(rmin, ok);
-        rmax = max(rmax, ok);
+        v.r_min_key = min(v.r_min_key, ok);
+        v.r_max_key = max(v.r_max_key, ok);
         if (key_out) key_out[i] = (uint32_t)k;
     }
-    for (uint32_t off = 32; off > 0; off >>= 1) {
-        max_read = max(max_read, (unsigned long long)__shfl_xor(max_read, off, 64));
-        max_key = max(max_key, (unsigned long long)__shfl_xor(max_key, off, 64));
-        rmin = min(rmin, (uint32_t)__shfl_xor(rmin, off, 64));
-        rmax = max(rmax, (uint32_t)__shfl_xor(rmax, off, 64));
-        bad += (uint32_t)__shfl_xor(bad, off, 64);
-    }
-    if ((threadIdx.x & 63u) == 0u) {
-        atomicMax(&out->max_read, max_read);
-        atomicMax(&out->max_key, max_key);
-        atomicMin(&out->r_min_key, rmin);
-        atomicMax(&out->r_max_key, rmax);
-        if (bad) atomicAdd(&out->bad, bad);
-    }
+    block_reduce_check(v);
+    if (threadIdx.x == 0u) part[blockIdx.x] = v;
+}
+
+// One block: folds `parts` partial results into *out.
+__global__ __launch_bounds__(256) void state_check_reduce_kernel(const StateCheck *__restrict__ part,
+                                                                 uint32_t parts,
+                                                                 StateCheck *__restrict__ out)
+{
+    StateCheck v{0ull, 0ull, 0xFFFFFFFFu, 0u, 0u, 0u};
+    for (uint32_t i = threadIdx.x; i < parts; i += 256u) combine_check(v, part[i]);
+    block_reduce_check(v);
+    if (threadIdx.x == 0u) *out = v;
 }
 
 hipError_t launch_state_check(const PhysConsts &c, const float4 *pos, const uint4 *vfme,
                               uint32_t *key_out, uint32_t n, StateCheck *out, hipStream_t stream)
 {
     if (n == 0) return hipSuccess;
-    const uint32_t blocks = std::min<uint32_t>((n + 255u) / 256u, 2048u);
+    const uint32_t blocks = std::min<uint32_t>((n + 255u) / 256u, kStateCheckParts);
     hipLaunchKernelGGL(state_check_kernel, dim3(blocks), dim3(256), 0, stream, c, pos, vfme,
-                       key_out, n, out);
+                       key_out, n, out + 1);
+    hipLaunchKernelGGL(state_check_reduce_kernel, dim3(1), dim3(256), 0, stream, out + 1, blocks,
+                       out);
     return hipGetLastError();
 }
 
