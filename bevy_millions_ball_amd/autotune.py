@@ -15,6 +15,7 @@ from typing import Optional, Tuple
 
 import numpy as np
 
+from ._lib import MbError
 from .physics import Physics, PhysicsSettings
 
 DEFAULT_CACHE = Path(os.environ.get("MB_TUNING_CACHE",
@@ -79,8 +80,11 @@ def tune(physics: Physics, cache: Optional[TuningCache] = None,
     key = scene_key(physics.settings, pos)
     hit = cache.get(key)
     if hit is not None:
-        physics.set_tuning(*hit)
-        return hit, True
+        try:
+            physics.set_tuning(*hit)
+            return hit, True
+        except MbError:
+            pass  # an entry from a library with other tuning choices: tune again
     best, ms = physics.autotune(frames_per_trial)
     cache.put(key, best, ms)
     return best, False

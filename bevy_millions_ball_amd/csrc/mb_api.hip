@@ -889,7 +889,7 @@ int mb_physics_set_tuning(mb_physics *p, const mb_tuning *t)
     if (!(t->scan_margin > 0.0f && t->scan_margin <= 4.0f))
         return fail(MB_ERR_INVALID_ARGUMENT, "scan_margin must be in (0, 4]");
     if (t->stage_pool != mb::kStagePool && t->stage_pool != mb::kStagePoolBig)
-        return fail(MB_ERR_INVALID_ARGUMENT, "stage_pool must be 896 or 1536");
+        return fail(MB_ERR_INVALID_ARGUMENT, "stage_pool must be 896 or 1408");
     if (t->row_sort_threads != 256u && t->row_sort_threads != mb::kSortThreads)
         return fail(MB_ERR_INVALID_ARGUMENT, "row_sort_threads must be 256 or 512");
     if (t->frame_graph > 1u) return fail(MB_ERR_INVALID_ARGUMENT, "frame_graph must be 0 or 1");

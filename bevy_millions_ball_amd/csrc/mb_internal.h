@@ -132,7 +132,7 @@ constexpr uint32_t kRowCap = 4096;      // spheres of one row sorted in LDS
 constexpr uint32_t kRowCapNarrow = 2048; // the same for the batched sort of sparse rows (256 threads)
 constexpr uint32_t kSortThreads = 512;  // threads of a row sort block (256 or 512)
 constexpr uint32_t kStagePool = 896;    // staged slots of a block's three neighbour-row windows
-constexpr uint32_t kStagePoolBig = 1536; // the same for dense scenes (fewer blocks per CU)
+constexpr uint32_t kStagePoolBig = 1408; // the same for dense scenes (fewer blocks per CU)
 
 // Row buckets.  The collide (or INIT) kernel of frame t files every sphere's
 // (column, slot) pair under the row of its next cell; the row sort of frame

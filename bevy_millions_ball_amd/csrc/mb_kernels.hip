@@ -1302,14 +1302,12 @@ __device__ __forceinline__ Ranges make_ranges(uint32_t lo0, uint32_t hi0, uint32
     return r;
 }
 
-// SPLIT (dense blocks, chosen per block): a chunk never crosses the end of a
-// range, so its staged candidates are consecutive in LDS and the scan steps a
-// pointer (13 VALU per candidate instead of 19).  Where ranges hold a few
-// candidates each (about one sphere per cell) the extra chunks cost more than
-// that saves, so the other form takes the three ranges in one chunk.
-template <bool SPRING, bool FROM_LDS, bool SPLIT>
+// A chunk takes up to 32 candidates across the three ranges (about one
+// sphere per cell, ranges hold a few candidates each); crowded waves take
+// walk_dense instead.
+template <bool SPRING, bool FROM_LDS>
 __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, uint32_t s,
-                                                const Ranges r, float rmax,
+                                                const Ranges r,
                                                 const float4 *__restrict__ s_pos,
                                                 const uint2 *__restrict__ s_vm,
                                                 const uint32_t *__restrict__ perm,
@@ -1332,32 +1330,16 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
     const float own_reach_w = own_reach * 1.0001f;
     const uint32_t t_self = s - r.slot0;       // own slot, always in the middle range
     const char *s_pos_b = reinterpret_cast<const char *>(s_pos);
-    // byte offset of candidate j's staged position (the unsplit form)
+    // byte offset of candidate j's staged position
     const uint32_t lb0 = r.lds0 * 16u, lb1 = r.lds1 * 16u, lb2 = r.lds2 * 16u;
     uint32_t j = 0;
     while (j < total) {
-        // split: candidate t of the chunk sits at LDS index t + lbase
-        const uint32_t bound = !SPLIT ? total : (j < r.n0 ? r.n0 : (j < r.n01 ? r.n01 : total));
-        const uint32_t end = min(j + 32u, bound);
-        const uint32_t lbase = r.lds0 + (j >= r.n0 ? r.lds1 : 0u) + (j >= r.n01 ? r.lds2 : 0u);
+        const uint32_t end = min(j + 32u, total);
         const float rx = a.px, ry = a.py, rz = a.pz;
         // candidate t of the chunk sets bit end - 1 - t: the first one is the
         // highest bit, so a shift-and-add builds the mask
         uint32_t near = 0;
-        if (FROM_LDS && SPLIT) {
-            // one reach for the chunk, from the largest radius staged: never
-            // narrower than a candidate's own
-            const float reach = __builtin_fmaf(rmax, 1.0001f, own_reach_w);
-            const float reach2 = reach * reach;
-            const char *p = s_pos_b + (j + lbase) * 16u;
-            const char *pe = s_pos_b + (end + lbase) * 16u;
-            for (; p < pe; p += 16) {
-                const float4 bp = *reinterpret_cast<const float4 *>(p);
-                const float dx = rx - bp.x, dy = ry - bp.y, dz = rz - bp.z;
-                const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                near = near + near + (d2 <= reach2 ? 1u : 0u);
-            }
-        } else if (FROM_LDS) {
+        if (FROM_LDS) {
             for (uint32_t t = j; t < end; ++t) {
                 const float4 bp = *reinterpret_cast<const float4 *>(
                     s_pos_b + (t * 16u + lb0 + (t >= r.n0 ? lb1 : 0u) + (t >= r.n01 ? lb2 : 0u)));
@@ -1383,7 +1365,7 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
             near ^= 0x80000000u >> lz;
             bool touch;
             if (FROM_LDS) {
-                const uint32_t l = SPLIT ? t + lbase : lds_of(t);
+                const uint32_t l = lds_of(t);
                 touch = collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
             } else {
                 const uint32_t bo = perm[slot_of(t)];
@@ -1402,9 +1384,16 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
     }
 }
 
-// The dense blocks' walk (the SPLIT form from LDS, one reach from the block's
-// largest radius), with the exact pass run over groups of up to four chunks
-// at a time: a wave then iterates as often as its busiest lane has near
+// The dense waves' walk.  A column of cells is 2H tall while the scan's
+// reach is about two radii, so most candidates are ruled out by height
+// alone: at staging the block files every pool index in a bitmap per height
+// slab (kSlabs slabs over [0, 2H]; s_slab[slab * sw + word], bit b of word w
+// = pool index 32 w + b), and a chunk of up to 32 consecutive candidates is
+// first narrowed to the pool indices whose slab meets the lane's window
+// [y - reach', y + reach'] (reach' = reach (1 + 10^-4), so every candidate
+// within the reach lies in it; the slab map is monotone in y), and only those
+// take the distance test.  The exact pass then runs over groups of up to four
+// chunks at a time: a wave iterates as often as its busiest lane has near
 // candidates in the group, where chunk by chunk it iterates as often as the
 // sum of each chunk's busiest lane.  Each chunk lies inside one range, so its
 // staged candidates are consecutive (candidate t at LDS index t + lb[g]).
@@ -1412,12 +1401,30 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
 // candidate; a contact that moves the sphere beyond the margin still restarts
 // the scan right after that candidate.
 constexpr int kDenseGroup = 4;
+#ifndef MB_DENSE_WALK_MIN
+#define MB_DENSE_WALK_MIN 48
+#endif
+constexpr uint32_t kDenseWalkMin = MB_DENSE_WALK_MIN;
+constexpr uint32_t kSlabs = 16;
+
+// words per slab of a pool of POOL entries: a chunk starting at pool index P
+// reads words P / 32 and P / 32 + 1, with P <= POOL - 1
+__host__ __device__ constexpr uint32_t slab_words(uint32_t pool) { return pool / 32u + 2u; }
+
+// slab of height y for inv_h = kSlabs / (2 H), clamped to [0, kSlabs - 1]
+// (NaN gives slab 0); non-decreasing in y
+__device__ __forceinline__ uint32_t slab_of(float y, float inv_h)
+{
+    return (uint32_t)fminf(fmaxf(y * inv_h, 0.0f), (float)(kSlabs - 1u));
+}
 
 template <bool SPRING>
 __device__ __forceinline__ void walk_dense(const PhysConsts &c, Agent &a, uint32_t s,
                                            const Ranges r, float rmax,
                                            const float4 *__restrict__ s_pos,
-                                           const uint2 *__restrict__ s_vm)
+                                           const uint2 *__restrict__ s_vm,
+                                           const uint32_t *__restrict__ s_slab, uint32_t sw,
+                                           float inv_h)
 {
     const uint32_t total = r.total;
     const float margin = c.scan_margin * a.scale;
@@ -1426,31 +1433,44 @@ __device__ __forceinline__ void walk_dense(const PhysConsts &c, Agent &a, uint32
     const float own_reach_w = (a.scale + margin) * 1.0001f;
     const float reach = __builtin_fmaf(rmax, 1.0001f, own_reach_w);
     const float reach2 = reach * reach;
+    const float reach_y = reach * 1.0001f;
     const uint32_t t_self = s - r.slot0;
-    const char *s_pos_b = reinterpret_cast<const char *>(s_pos);
     uint32_t j = 0;
     while (j < total) {
         const float rx = a.px, ry = a.py, rz = a.pz;
-        uint32_t mask[kDenseGroup], end[kDenseGroup], lb[kDenseGroup];
+        const uint32_t slo = slab_of(ry - reach_y, inv_h), shi = slab_of(ry + reach_y, inv_h);
+        // chunk g: candidates [first[g], first[g] + 32) at LDS index t + lb[g];
+        // bit i of mask[g] is candidate first[g] + i
+        uint32_t mask[kDenseGroup], first[kDenseGroup], lb[kDenseGroup];
         uint32_t b = j;
 #pragma unroll
         for (int g = 0; g < kDenseGroup; ++g) {
             const uint32_t bound = b < r.n0 ? r.n0 : (b < r.n01 ? r.n01 : total);
             const uint32_t e = min(b + 32u, bound);  // e == b once b reaches total
             lb[g] = r.lds0 + (b >= r.n0 ? r.lds1 : 0u) + (b >= r.n01 ? r.lds2 : 0u);
+            first[g] = b;
             uint32_t m = 0;
-            const char *p = s_pos_b + (b + lb[g]) * 16u;
-            const char *pe = s_pos_b + (e + lb[g]) * 16u;
-            for (; p < pe; p += 16) {
-                const float4 bp = *reinterpret_cast<const float4 *>(p);
-                const float dx = rx - bp.x, dy = ry - bp.y, dz = rz - bp.z;
-                const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-                m = m + m + (d2 <= reach2 ? 1u : 0u);
+            if (e > b) {
+                const uint32_t P = b + lb[g];  // pool index of candidate b
+                const uint32_t *wp = s_slab + (P >> 5);
+                uint32_t ym = 0;
+                for (uint32_t sl = slo; sl <= shi; ++sl) {
+                    const uint32_t *row = wp + sl * sw;
+                    ym |= __builtin_amdgcn_alignbit(row[1], row[0], P & 31u);
+                }
+                const uint32_t cnt = e - b;
+                ym &= cnt >= 32u ? 0xFFFFFFFFu : ((1u << cnt) - 1u);
+                if (t_self >= b && t_self < e) ym &= ~(1u << (t_self - b));
+                while (ym) {
+                    const uint32_t i = (uint32_t)__builtin_ctz(ym);
+                    ym &= ym - 1u;
+                    const float4 bp = s_pos[P + i];
+                    const float dx = rx - bp.x, dy = ry - bp.y, dz = rz - bp.z;
+                    const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                    m |= (d2 <= reach2 ? 1u : 0u) << i;
+                }
             }
-            // candidate t of chunk g sets bit e - 1 - t
-            if (t_self >= b && t_self < e) m &= ~(1u << (e - 1u - t_self));
             mask[g] = m;
-            end[g] = e;
             b = e;
         }
         uint32_t next = b;
@@ -1459,11 +1479,11 @@ __device__ __forceinline__ void walk_dense(const PhysConsts &c, Agent &a, uint32
             const bool h0 = mask[0] != 0u, h1 = mask[1] != 0u, h2 = mask[2] != 0u;
             const uint32_t m = h0 ? mask[0] : (h1 ? mask[1] : (h2 ? mask[2] : mask[3]));
             if (m == 0u) break;
-            const uint32_t e = h0 ? end[0] : (h1 ? end[1] : (h2 ? end[2] : end[3]));
+            const uint32_t f = h0 ? first[0] : (h1 ? first[1] : (h2 ? first[2] : first[3]));
             const uint32_t l0 = h0 ? lb[0] : (h1 ? lb[1] : (h2 ? lb[2] : lb[3]));
-            const uint32_t lz = (uint32_t)__builtin_clz(m);
-            const uint32_t t = e - 32u + lz;
-            const uint32_t bit = 0x80000000u >> lz;
+            const uint32_t i = (uint32_t)__builtin_ctz(m);
+            const uint32_t t = f + i;
+            const uint32_t bit = 1u << i;
             mask[0] ^= h0 ? bit : 0u;
             mask[1] ^= (!h0 && h1) ? bit : 0u;
             mask[2] ^= (!h0 && !h1 && h2) ? bit : 0u;
@@ -1489,20 +1509,24 @@ __device__ __forceinline__ void walk_dense(const PhysConsts &c, Agent &a, uint32
 // ranges (physics.wgsl:119-206) in the shader's order.  The block's first and
 // last cells come from the sorted keys, so the window bounds are uniform
 // loads issued at launch and one barrier separates staging from the walk.
-// DENSE: blocks whose slots hold 2 or more spheres per cell take the split
-// candidate scan (a kernel variant of its own, so that scenes near one sphere
-// per cell run the code without it).
+// DENSE: waves with a lane of more than kDenseWalkMin candidates take
+// walk_dense, and the block files its pool in the height-slab bitmaps (a
+// kernel variant of its own, so that scenes near one sphere per cell run the
+// code without it).
 template <bool SPRING, uint32_t POOL, bool PRIO = false, bool DENSE = false>
 __device__ __forceinline__ void physics_step_body(
     const PhysConsts &c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
     uint32_t *__restrict__ keys, const float4 *__restrict__ pos_i,
     const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
     const RowBuckets &rb, uint32_t set_out, uint32_t n, uint32_t blk,
-    float4 *__restrict__ s_pos, uint2 *__restrict__ s_vm)
+    float4 *__restrict__ s_pos, uint2 *__restrict__ s_vm, uint32_t *__restrict__ s_slab)
 {
     __shared__ BlockFiling s_file;
     if (PRIO) __builtin_amdgcn_s_setprio(2);
     const uint32_t tid = threadIdx.x;
+    constexpr uint32_t sw = slab_words(POOL);
+    if (DENSE)
+        for (uint32_t i = tid; i < kSlabs * sw; i += 256u) s_slab[i] = 0u;
     const uint32_t s0 = blk * 256u;
     const uint32_t last = min(n - s0, 256u) - 1u;
     const uint32_t s = s0 + min(tid, last);  // idle threads mirror the last slot
@@ -1554,9 +1578,10 @@ __device__ __forceinline__ void physics_step_body(
     const uint32_t wbase[3] = {0u, wcnt[0], wcnt[0] + wcnt[1]};
     const uint32_t wtot = wcnt[0] + wcnt[1] + wcnt[2];
     const bool staged = wtot <= POOL;
-    // dense block: its slots span few cells (2 or more spheres per cell)
-    const bool split = DENSE && staged && (klast - kfirst + 1u) * 2u <= last + 1u;
     __shared__ float s_wave_rmax[4];
+    // slab height 2H / kSlabs
+    const float inv_h = (float)kSlabs / (2.0f * c.half_map_height);
+    if (DENSE) __syncthreads();  // s_slab cleared before the staging files into it
     if (staged) {
         // one pass over the pool: index j of window w holds slot
         // wlo[w] + j - wbase[w] (u32 arithmetic, wrapping)
@@ -1568,31 +1593,42 @@ __device__ __forceinline__ void physics_step_body(
                 perm[j + wlo[0] + (j >= wbase[1] ? d1 : 0u) + (j >= wbase[2] ? d2 : 0u)];
             const float4 bp = pos_i[bo];
             s_pos[j] = bp;
-            if (DENSE) rmax = fmaxf(rmax, bp.w);
+            if (DENSE) {
+                rmax = fmaxf(rmax, bp.w);
+                atomicOr(&s_slab[slab_of(bp.y, inv_h) * sw + (j >> 5)], 1u << (j & 31u));
+            }
             if (!SPRING) s_vm[j] = vel_mass_of(vfme_i[bo]);
         }
-        if (split) {
+        if (DENSE) {
             // the largest radius in the windows, for the dense scan's reach
             for (uint32_t o = 32; o > 0; o >>= 1) rmax = fmaxf(rmax, __shfl_xor(rmax, o, 64));
             if ((tid & 63u) == 0u) s_wave_rmax[tid >> 6] = rmax;
         }
     }
+    const Ranges rg = make_ranges(lo0, hi0, lo1, hi1, lo2, hi2, wlo[0], wlo[1], wlo[2], wbase[1],
+                                  wbase[2]);
+    // dense wave: one of its lanes has more than kDenseWalkMin candidates
+    // (idle lanes mirror the last slot, so every lane takes part)
+    bool split = false;
+    if (DENSE && staged) {
+        uint32_t most = rg.total;
+        for (uint32_t o = 32; o > 0; o >>= 1) most = max(most, (uint32_t)__shfl_xor(most, o, 64));
+        split = most > kDenseWalkMin;
+    }
     __syncthreads();
     if (PRIO) __builtin_amdgcn_s_setprio(0);
     if (!valid) return;
 
-    const Ranges rg = make_ranges(lo0, hi0, lo1, hi1, lo2, hi2, wlo[0], wlo[1], wlo[2], wbase[1],
-                                  wbase[2]);
     verlet(c, a);
     const float ox = a.px, oy = a.py, oz = a.pz;
     if (DENSE && split) {
         const float rmax = fmaxf(fmaxf(s_wave_rmax[0], s_wave_rmax[1]),
                                  fmaxf(s_wave_rmax[2], s_wave_rmax[3]));
-        walk_dense<SPRING>(c, a, s, rg, rmax, s_pos, s_vm);
+        walk_dense<SPRING>(c, a, s, rg, rmax, s_pos, s_vm, s_slab, sw, inv_h);
     } else if (staged) {
-        walk_candidates<SPRING, true, false>(c, a, s, rg, 0.0f, s_pos, s_vm, perm, pos_i, vfme_i);
+        walk_candidates<SPRING, true>(c, a, s, rg, s_pos, s_vm, perm, pos_i, vfme_i);
     } else {
-        walk_candidates<SPRING, false, false>(c, a, s, rg, 0.0f, s_pos, s_vm, perm, pos_i, vfme_i);
+        walk_candidates<SPRING, false>(c, a, s, rg, s_pos, s_vm, perm, pos_i, vfme_i);
     }
     jitter(a, s, ox, oy, oz);
     const uint32_t next = finish_agent<SPRING, true>(c, a, s, pos_o, vfme_o, keys);
@@ -1611,9 +1647,10 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
 {
     __shared__ float4 s_pos[POOL];  // the three windows back to back
     __shared__ uint2 s_vm[POOL];    // their velocity and mass halves
+    __shared__ uint32_t s_slab[DENSE ? kSlabs * slab_words(POOL) : 1u];  // height-slab bitmaps
     physics_step_body<SPRING, POOL, PRIO, DENSE>(c, perm, off, keys, pos_i, vfme_i, pos_o, vfme_o,
                                                  rb, set_out, n, xcd_block(blockIdx.x, gridDim.x),
-                                                 s_pos, s_vm);
+                                                 s_pos, s_vm, s_slab);
 }
 
 // INIT frame of the frame path: the sort of the all-zero keys is the
@@ -1853,7 +1890,7 @@ __global__ __launch_bounds__(256) void batch_physics_step_kernel(
     const uint32_t p = (uint32_t)time & 1u;
     physics_step_body<SPRING, kStagePool>(S.c, gptr(S.perm), gptr(S.off), gptr(S.keys), gptr(S.pos[p]),
                               gptr(S.vfme[p]), gptr(S.pos[p ^ 1u]), gptr(S.vfme[p ^ 1u]), rb,
-                              p ^ 1u, S.n, m.y, s_pos, s_vm);
+                              p ^ 1u, S.n, m.y, s_pos, s_vm, nullptr);
 }
 
 __global__ __launch_bounds__(64) void batch_advance_kernel(uint32_t *__restrict__ counter)

@@ -137,18 +137,18 @@ int mb_physics_download_state(const mb_physics *physics, float *position_with_sc
  * margin as a fraction of a sphere's radius (wider: more candidates reach
  * the exact test; narrower: the scan restarts more often), in (0, 4];
  * stage_pool is the number of neighbour slots a collide block stages in LDS,
- * 896 (7 blocks per CU) or 1536 (4 blocks per CU; fewer blocks of dense
- * scenes fall back to reading neighbours from global memory);
+ * 896 or 1408 (fewer resident blocks; fewer blocks of dense scenes fall back
+ * to reading neighbours from global memory);
  * row_sort_threads is the width of a row sort block, 512 (rows of up to 4096
  * spheres sorted in LDS) or 256 (up to 2048; twice as many rows resident);
  * frame_graph (0 / 1) lets mb_physics_step replay runs of LOOP frames as a
  * captured hipGraph of two frames instead of launching each kernel (never
  * while profiling; the frame after INIT and frames on the stage kernels are
  * launched directly); dense_scan (0 / 1) runs the collide variant in which
- * blocks holding 2 or more spheres per cell scan their candidates range by
- * range with one reach from the block's largest radius (fewer instructions
- * per candidate where cells are crowded, a little slower near one sphere per
- * cell).  Defaults: 1.0, 896, 512, 0 and 0.  The autotuner searches
+ * waves with a lane of more than 48 candidates scan them range by range,
+ * with one reach from the block's largest radius, testing only the
+ * candidates whose height slab meets the lane's reach (far fewer tests where
+ * cells are crowded, a little slower near one sphere per cell).  Defaults: 1.0, 896, 512, 0 and 0.  The autotuner searches
  * scan_margin x stage_pool x row_sort_threads x dense_scan, then times its
  * winner with the other frame_graph choice. */
 typedef struct mb_tuning {

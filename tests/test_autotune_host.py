@@ -29,8 +29,8 @@ def test_cache_round_trip(tmp_path):
     path = tmp_path / "t.json"
     c = TuningCache(path)
     assert c.get("k") is None
-    c.put("k", (0.5, 1536, 256, 1), 0.123)
-    assert TuningCache(path).get("k") == (0.5, 1536, 256, 1)
+    c.put("k", (0.5, 1408, 256, 1), 0.123)
+    assert TuningCache(path).get("k") == (0.5, 1408, 256, 1)
     path.write_text("not json")
     assert TuningCache(path).get("k") is None  # a damaged cache is ignored
 
@@ -52,7 +52,7 @@ class FakePhysics:
 
     def autotune(self, frames):
         self.autotuned += 1
-        self.applied = (0.25, 1536, 256, 1)
+        self.applied = (0.25, 1408, 256, 1)
         return self.applied, 1.5
 
 
@@ -61,7 +61,7 @@ def test_tune_times_once_then_uses_the_cache(tmp_path):
     pos, _ = generate(cfg, seed=0)
     cache = TuningCache(tmp_path / "t.json")
     first = FakePhysics(cfg.settings(), pos)
-    assert tune(first, cache) == ((0.25, 1536, 256, 1), False) and first.autotuned == 1
+    assert tune(first, cache) == ((0.25, 1408, 256, 1), False) and first.autotuned == 1
     second = FakePhysics(cfg.settings(), pos)
-    assert tune(second, TuningCache(tmp_path / "t.json")) == ((0.25, 1536, 256, 1), True)
-    assert second.autotuned == 0 and second.applied == (0.25, 1536, 256, 1)
+    assert tune(second, TuningCache(tmp_path / "t.json")) == ((0.25, 1408, 256, 1), True)
+    assert second.autotuned == 0 and second.applied == (0.25, 1408, 256, 1)
