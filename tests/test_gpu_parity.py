@@ -239,6 +239,50 @@ def test_long_run_dense_scan_full_size(gpu, oracle):
         compare_frames(g, r, 60, check_every=20)
 
 
+@pytest.mark.parametrize("spring", [False, True])
+@pytest.mark.parametrize("half_height", [1, 8, 64])
+def test_dense_scan_mixed_radii_and_heights(gpu, oracle, spring, half_height):
+    """The dense-scan walk tests only the candidates whose height slab (16
+    slabs over the box height) meets the lane's reach, which comes from the
+    block's largest radius.  Clumps of up to ~90 spheres per cell (up to
+    ~700 candidates a sphere) with radii from 0.1 to 0.9, in boxes 2, 16 and
+    128 tall (slabs far thinner than the reach, about as tall, far taller),
+    must give the oracle's frames, with the 1408-slot pool (nearly every block
+    staged) and with the 896-slot one (most blocks overflow it and walk from
+    global memory)."""
+    rng = np.random.default_rng(31 + half_height + (7 if spring else 0))
+    n, grids, gs = 16_384, 64, 2
+    ps = PhysicsSettings(number_of_grids_one_dimension=grids, grid_size=gs,
+                         half_map_height=half_height, enable_spring=spring, e_agent=0.8,
+                         e_envir=0.8, time_delta=1 / 120, _max_number_of_agents=n)
+    hms = grids * gs // 2
+    centres = rng.uniform(-hms / 2, hms / 2, (4, 2))
+    xz = centres[rng.integers(0, 4, n)] + rng.normal(0.0, 8.0, (n, 2))
+    xz = np.clip(xz, -hms + 1.0, hms - 1.0)
+    r = rng.uniform(0.1, 0.9, n).astype(np.float32)
+    pos = np.empty((n, 4), np.float32)
+    pos[:, 0], pos[:, 2] = xz[:, 0], xz[:, 1]
+    pos[:, 1] = rng.uniform(r, 2 * half_height - r)
+    pos[:, 3] = r
+    vf = pack_vfme(rng.uniform(-1, 1, (n, 3)).astype(np.float32), np.zeros((n, 3), np.float32),
+                   rng.uniform(0.5, 2.0, n).astype(np.float32))
+    ref = oracle.World(ps)
+    ref.upload(pos, vf)
+    ref.step(6)
+    want = ref.download(), ref.debug()["next_keys"]
+    ref.close()
+    for tuning in [(0.25, 1408, 512, 1), (1.0, 896, 512, 1)]:
+        print(f"[dense-radii] spring={spring} H={half_height} {tuning}", flush=True)
+        with Physics(ps) as g:
+            g.set_tuning(*tuning)
+            g.upload_state(pos, vf)
+            g.run(6)
+            pg, vg = g.download_state()
+            assert np.array_equal(pg.view(np.uint32), want[0][0].view(np.uint32)), tuning
+            assert np.array_equal(vg, want[0][1]), tuning
+            assert np.array_equal(g.debug()["next_keys"], want[1]), tuning
+
+
 def test_largest_grid_bit_exact(gpu, oracle):
     """The largest grid the keys allow (G = 4096, G^2 = 2^24 cells) with
     N = G^2 spheres: ~4096 spheres per row, so about half the rows exceed the
