@@ -262,6 +262,7 @@ struct BatchPlan {
     uint32_t blk_spring_count;
     uint32_t max_grids;       // largest G among the scenes (row sort LDS)
     uint32_t max_row_mean;    // largest ceil(N / G) among the scenes (row sort width)
+    uint32_t max_row_peak;    // largest row of any scene's submitted state (row sort width)
 };
 
 // Enqueues frame k of the plan for every scene (INIT for those at time 0).

@@ -1923,12 +1923,13 @@ hipError_t launch_batch_frame(const BatchPlan &p, uint32_t k, bool from_counter,
     const uint32_t *counter = from_counter ? p.frame_counter : nullptr;
     if (p.scene_count == 0) return hipSuccess;
     if (p.rows) {
-        // rows of at most 1024 spheres on average: 256-thread blocks with
-        // room for 2048 spheres of a row in LDS (fewer registers and less
-        // LDS, so more rows resident at once; each row costs about the same
-        // latency, and a longer row is sorted in global memory); denser
-        // rows: kSortThreads and kRowCap
-        if (p.max_row_mean <= 1024u)
+        // rows of at most 1024 spheres on average, none above 1536 when
+        // submitted: 256-thread blocks with room for 2048 spheres of a row
+        // in LDS (fewer registers and less LDS, so more rows resident at
+        // once; each row costs about the same latency, and a longer row is
+        // sorted in global memory); denser or clumped rows: kSortThreads and
+        // kRowCap
+        if (p.max_row_mean <= 1024u && p.max_row_peak <= 1536u)
             hipLaunchKernelGGL((batch_row_sort_kernel<256, kRowCapNarrow>), dim3(p.rows), dim3(256),
                                (row_sort_lds<256, kRowCapNarrow>(p.max_grids)), stream, p.scenes,
                                p.row_map, k, counter);

@@ -211,6 +211,7 @@ struct Request {
     mb_settings settings{};
     mb::PhysConsts consts{};
     uint32_t n = 0, g = 0, steps = 0;
+    uint32_t row_peak = 0;      // spheres in the fullest row of the submitted state
     bool keep_debug = false;
     int state = MB_REQUEST_QUEUED;
     SceneLayout layout{};
@@ -471,12 +472,13 @@ int upload_plan(mb_server *s)
     std::vector<mb::SceneDesc> descs;
     std::vector<uint2> rows, plain, spring;
     descs.reserve(count);
-    uint32_t max_g = 1, max_rm = 1;
+    uint32_t max_g = 1, max_rm = 1, max_peak = 0;
     for (uint32_t i = 0; i < count; ++i) {
         const Request *r = s->running[i];
         descs.push_back(r->desc());
         max_g = std::max(max_g, r->g);
         max_rm = std::max(max_rm, (r->n + r->g - 1u) / r->g);
+        max_peak = std::max(max_peak, r->row_peak);
         for (uint32_t g = 0; g < r->g; ++g) rows.push_back(make_uint2(i, g));
         auto &blk = r->consts.enable_spring ? spring : plain;
         for (uint32_t b = 0; b < (r->n + 255u) / 256u; ++b) blk.push_back(make_uint2(i, b));
@@ -535,6 +537,7 @@ int upload_plan(mb_server *s)
     p.blk_spring_count = (uint32_t)spring.size();
     p.max_grids = max_g;
     p.max_row_mean = max_rm;
+    p.max_row_peak = max_peak;
 
     if (s->config.use_graph) {
         if (s->exec) {
@@ -704,6 +707,7 @@ int mb_server_submit(mb_server *s, const mb_settings *settings, const float *pos
                      const uint32_t *vfme, uint32_t steps, int keep_debug, uint64_t *id)
 {
     if (!s || !settings || !pos || !vfme || !id) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    std::vector<uint32_t> rows;  // spheres per row of the submitted state
     int rc = validate_scene(*settings);
     if (rc != MB_OK) return rc;
     const uint32_t n = settings->max_number_of_agents;
@@ -724,11 +728,14 @@ int mb_server_submit(mb_server *s, const mb_settings *settings, const float *pos
         // states through the stage kernels).
         const mb::PhysConsts c = mb::compile_consts(*settings);
         float r_min = pos[3], r_max = pos[3];
+        rows.assign(g, 0u);
         for (uint32_t i = 0; i < n; ++i) {
             const float *q = pos + 4u * (size_t)i;
-            if (mb::host_cell(c, q[0]) >= g || mb::host_cell(c, q[2]) >= g)
+            const uint32_t cz = mb::host_cell(c, q[2]);
+            if (mb::host_cell(c, q[0]) >= g || cz >= g)
                 return fail(MB_ERR_INVALID_ARGUMENT,
                             "sphere " + std::to_string(i) + " lies outside the grid (serving needs every cell in it)");
+            rows[cz] += 1u;
             r_min = std::min(r_min, q[3]);
             r_max = std::max(r_max, q[3]);
         }
@@ -746,6 +753,7 @@ int mb_server_submit(mb_server *s, const mb_settings *settings, const float *pos
     r->consts = mb::compile_consts(*settings);
     r->n = n;
     r->g = g;
+    r->row_peak = rows.empty() ? 0u : *std::max_element(rows.begin(), rows.end());
     r->steps = steps;
     r->keep_debug = keep_debug != 0;
     r->layout = layout;
