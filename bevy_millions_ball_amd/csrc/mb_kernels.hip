@@ -973,10 +973,13 @@ __device__ __forceinline__ uint32_t block_scan_lds(uint32_t *vals, uint32_t coun
 }
 
 // Files (column, slot) under `row` in a bucket set whose counters are STRIDE
-// words apart (fill[row * STRIDE]): the lanes of a wave that share a row take
-// consecutive places with one returning atomic per distinct row, the leaders
-// of all rows issuing theirs at once.  A pair beyond the bucket's capacity
-// goes to the spill list.  Every active lane files one pair.
+// words apart (fill[row * STRIDE]).  Where the wave's rows come in runs (input
+// in sorted order), the lanes that share a row take consecutive places with
+// one returning atomic per distinct row, the leaders of all rows issuing
+// theirs at once; where most neighbouring lanes differ (input in upload
+// order), every lane takes its own.  The row sort does not depend on the
+// order inside a bucket.  A pair beyond the bucket's capacity goes to the
+// spill list.  Every active lane files one pair.
 template <uint32_t STRIDE>
 __device__ __forceinline__ void bucket_file(const RowBuckets &rb, uint32_t *__restrict__ fill,
                                             uint32_t *__restrict__ spill_count, uint32_t row,
@@ -985,21 +988,32 @@ __device__ __forceinline__ void bucket_file(const RowBuckets &rb, uint32_t *__re
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = lanemask_lt(lane);
     uint64_t todo = __ballot(1);
-    uint32_t leader_lane = lane, rank = 0, count = 0;
-    while (todo) {
-        const uint32_t leader = (uint32_t)__ffsll((long long)todo) - 1u;
-        const uint32_t lr = (uint32_t)__shfl((int)row, (int)leader, 64);
-        const uint64_t peers = __ballot(row == lr);
-        if (row == lr) {
-            leader_lane = leader;
-            rank = (uint32_t)__popcll(peers & lt);
+    // rows that differ from the previous active lane's: few where the input
+    // is in sorted order (a re-key), nearly all in upload order (INIT)
+    const uint32_t prev_row = (uint32_t)__shfl_up((int)row, 1, 64);
+    const bool starts = lane == 0u || !((todo >> (lane - 1u)) & 1ull) || prev_row != row;
+    uint32_t at;
+    if (__popcll(__ballot(starts)) > 16) {
+        // mostly distinct rows: one atomic per sphere, all issued together
+        at = atomicAdd(&fill[row * STRIDE], 1u);
+    } else {
+        // one atomic per distinct row, its spheres ranked by lane
+        uint32_t leader_lane = lane, rank = 0, count = 0;
+        while (todo) {
+            const uint32_t leader = (uint32_t)__ffsll((long long)todo) - 1u;
+            const uint32_t lr = (uint32_t)__shfl((int)row, (int)leader, 64);
+            const uint64_t peers = __ballot(row == lr);
+            if (row == lr) {
+                leader_lane = leader;
+                rank = (uint32_t)__popcll(peers & lt);
+            }
+            if (lane == leader) count = (uint32_t)__popcll(peers);
+            todo &= ~peers;
         }
-        if (lane == leader) count = (uint32_t)__popcll(peers);
-        todo &= ~peers;
+        uint32_t first = 0;
+        if (count) first = atomicAdd(&fill[row * STRIDE], count);
+        at = (uint32_t)__shfl((int)first, (int)leader_lane, 64) + rank;
     }
-    uint32_t first = 0;
-    if (count) first = atomicAdd(&fill[row * STRIDE], count);
-    const uint32_t at = (uint32_t)__shfl((int)first, (int)leader_lane, 64) + rank;
     if (at < rb.cap) {
         rb.bucket[(size_t)row * rb.cap + at] = ((uint64_t)col << 32) | slot;
     } else {
