@@ -917,82 +917,74 @@ int mb_physics_autotune(mb_physics *p, uint32_t frames, mb_tuning *best, double 
     if (rc != MB_OK) return rc;
     mb_tuning keep;
     (void)mb_physics_get_tuning(p, &keep);
-    const float margins[] = {1.0f, 0.5f, 0.25f};
-    const uint32_t pools[] = {mb::kStagePool, mb::kStagePoolBig};
-    const uint32_t sorts[] = {mb::kSortThreads, 256u};
-    const uint32_t dense[] = {0u, 1u};
-    mb_tuning winner = keep;
-    double winner_ms = 0.0;
-    bool have = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     MB_HIP(hipEventCreate(&e0));
     MB_HIP(hipEventCreate(&e1));
+    // one trial: the checkpointed state, tuning t, fr LOOP frames timed
+    auto trial = [&](const mb_tuning &t, uint32_t fr, double &per) -> int {
+        int rc = mb_physics_load_checkpoint(p, ck.data(), ck.size());
+        if (rc == MB_OK) rc = mb_physics_set_tuning(p, &t);
+        if (rc == MB_OK && p->time < 0) rc = mb_physics_step(p, 1, nullptr);  // INIT
+        hipError_t e = hipSuccess;
+        if (rc == MB_OK) e = hipEventRecord(e0, nullptr);
+        if (rc == MB_OK && e == hipSuccess) rc = mb_physics_step(p, fr, nullptr);
+        if (rc == MB_OK && e == hipSuccess) e = hipEventRecord(e1, nullptr);
+        if (rc == MB_OK && e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (rc == MB_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (rc == MB_OK && e != hipSuccess) rc = fail_hip(e, "autotune timing");
+        per = (double)ms / fr;
+        return rc;
+    };
+    auto give_up = [&](int rc) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)mb_physics_load_checkpoint(p, ck.data(), ck.size());
+        (void)mb_physics_set_tuning(p, &keep);
+        return rc;
+    };
+    // every candidate for `frames` frames, then the four fastest again over
+    // four times as many (short trials cannot tell apart choices a few
+    // per cent apart), then the winner against its other frame_graph choice
+    const float margins[] = {1.0f, 0.5f, 0.25f, 0.0625f};
+    const uint32_t pools[] = {mb::kStagePool, mb::kStagePoolBig};
+    const uint32_t sorts[] = {mb::kSortThreads, 256u};
+    const uint32_t dense[] = {0u, 1u};
+    std::vector<std::pair<double, mb_tuning>> coarse;
     for (uint32_t ds : dense)
-    for (uint32_t sort : sorts)
-    for (uint32_t pool : pools) {
-        for (float margin : margins) {
-            const mb_tuning t{margin, pool, sort, keep.frame_graph, ds};
-            rc = mb_physics_load_checkpoint(p, ck.data(), ck.size());
-            if (rc == MB_OK) rc = mb_physics_set_tuning(p, &t);
-            if (rc == MB_OK && p->time < 0) rc = mb_physics_step(p, 1, nullptr);  // INIT
-            hipError_t e = hipSuccess;
-            if (rc == MB_OK) e = hipEventRecord(e0, nullptr);
-            if (rc == MB_OK && e == hipSuccess) rc = mb_physics_step(p, frames, nullptr);
-            if (rc == MB_OK && e == hipSuccess) e = hipEventRecord(e1, nullptr);
-            if (rc == MB_OK && e == hipSuccess) e = hipEventSynchronize(e1);
-            float ms = 0.0f;
-            if (rc == MB_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-            if (rc == MB_OK && e != hipSuccess) rc = fail_hip(e, "autotune timing");
-            if (rc != MB_OK) {
-                (void)hipEventDestroy(e0);
-                (void)hipEventDestroy(e1);
-                (void)mb_physics_load_checkpoint(p, ck.data(), ck.size());
-                (void)mb_physics_set_tuning(p, &keep);
-                return rc;
-            }
-            const double per = (double)ms / frames;
-            if (!have || per < winner_ms) {
-                winner = t;
-                winner_ms = per;
-                have = true;
-            }
+        for (uint32_t sort : sorts)
+            for (uint32_t pool : pools)
+                for (float margin : margins) {
+                    const mb_tuning t{margin, pool, sort, keep.frame_graph, ds};
+                    double per = 0.0;
+                    const int rc = trial(t, frames, per);
+                    if (rc != MB_OK) return give_up(rc);
+                    coarse.emplace_back(per, t);
+                }
+    std::stable_sort(coarse.begin(), coarse.end(),
+                     [](const std::pair<double, mb_tuning> &x,
+                        const std::pair<double, mb_tuning> &y) { return x.first < y.first; });
+    const uint32_t fr = std::max(4u * frames, 8u);
+    mb_tuning winner = coarse[0].second;
+    double winner_ms = 0.0;
+    for (size_t i = 0; i < std::min<size_t>(4, coarse.size()); ++i) {
+        double per = 0.0;
+        const int rc = trial(coarse[i].second, fr, per);
+        if (rc != MB_OK) return give_up(rc);
+        if (i == 0 || per < winner_ms) {
+            winner = coarse[i].second;
+            winner_ms = per;
         }
     }
     {
-        // the winner with the other frame_graph choice: graph replays need
-        // runs of frames, so this trial runs at least four
+        // graph replays need runs of frames: fr >= 8
         mb_tuning t = winner;
         t.frame_graph = winner.frame_graph ? 0u : 1u;
-        const uint32_t fr = frames < 4u ? 4u : frames;
-        rc = mb_physics_load_checkpoint(p, ck.data(), ck.size());
-        if (rc == MB_OK) rc = mb_physics_set_tuning(p, &t);
-        if (rc == MB_OK && p->time < 0) rc = mb_physics_step(p, 1, nullptr);
-        double ms_base = 0.0, ms_flip = 0.0;
-        for (int pass = 0; pass < 2 && rc == MB_OK; ++pass) {
-            // pass 0: the flipped choice; pass 1: the winner again over as many frames
-            const mb_tuning use = pass == 0 ? t : winner;
-            if (pass == 1) {
-                rc = mb_physics_load_checkpoint(p, ck.data(), ck.size());
-                if (rc == MB_OK) rc = mb_physics_set_tuning(p, &use);
-                if (rc == MB_OK && p->time < 0) rc = mb_physics_step(p, 1, nullptr);
-            }
-            hipError_t e = hipSuccess;
-            if (rc == MB_OK) e = hipEventRecord(e0, nullptr);
-            if (rc == MB_OK && e == hipSuccess) rc = mb_physics_step(p, fr, nullptr);
-            if (rc == MB_OK && e == hipSuccess) e = hipEventRecord(e1, nullptr);
-            if (rc == MB_OK && e == hipSuccess) e = hipEventSynchronize(e1);
-            float ms = 0.0f;
-            if (rc == MB_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-            if (rc == MB_OK && e != hipSuccess) rc = fail_hip(e, "autotune timing");
-            (pass == 0 ? ms_flip : ms_base) = (double)ms / fr;
-        }
-        if (rc != MB_OK) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            (void)mb_physics_load_checkpoint(p, ck.data(), ck.size());
-            (void)mb_physics_set_tuning(p, &keep);
-            return rc;
-        }
+        double ms_flip = 0.0, ms_base = 0.0;
+        int rc = trial(t, fr, ms_flip);
+        if (rc == MB_OK) rc = trial(winner, fr, ms_base);
+        if (rc != MB_OK) return give_up(rc);
+        winner_ms = ms_base;
         if (ms_flip < ms_base) {
             winner = t;
             winner_ms = ms_flip;

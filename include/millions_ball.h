@@ -148,9 +148,10 @@ int mb_physics_download_state(const mb_physics *physics, float *position_with_sc
  * waves with a lane of more than 48 candidates scan them range by range,
  * with one reach from the block's largest radius, testing only the
  * candidates whose height slab meets the lane's reach (far fewer tests where
- * cells are crowded, a little slower near one sphere per cell).  Defaults: 1.0, 896, 512, 0 and 0.  The autotuner searches
- * scan_margin x stage_pool x row_sort_threads x dense_scan, then times its
- * winner with the other frame_graph choice. */
+ * cells are crowded, a little slower near one sphere per cell).
+ * Defaults: 1.0, 896, 512, 0 and 0.  The autotuner searches scan_margin
+ * (1, 1/2, 1/4, 1/16) x stage_pool x row_sort_threads x dense_scan, then
+ * times its winner with the other frame_graph choice. */
 typedef struct mb_tuning {
     float scan_margin;
     uint32_t stage_pool;

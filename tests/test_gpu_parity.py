@@ -228,14 +228,14 @@ def test_long_run_bit_exact_full_size(gpu, oracle, name, frames):
 
 def test_long_run_dense_scan_full_size(gpu, oracle):
     """The spring-clumps scene at the point the autotuner picks for it (margin
-    0.25, 1408-slot pool, dense scan: height-slab filtered walks, and in the
+    1/16, 1408-slot pool, dense scan: height-slab filtered walks, and in the
     first frames many rescans after contacts that move a sphere beyond the
     margin), 60 frames at full size, compared every 20."""
     cfg = CONFIGS["spring_clumps_1m"]
     print("[long-dense] spring_clumps_1m frames=60", flush=True)
     g, r = make_pair(oracle, cfg, seed=7)
     with g:
-        g.set_tuning(0.25, 1408, 512, 1)
+        g.set_tuning(0.0625, 1408, 512, 1)
         compare_frames(g, r, 60, check_every=20)
 
 
@@ -271,7 +271,7 @@ def test_dense_scan_mixed_radii_and_heights(gpu, oracle, spring, half_height):
     ref.step(6)
     want = ref.download(), ref.debug()["next_keys"]
     ref.close()
-    for tuning in [(0.25, 1408, 512, 1), (1.0, 896, 512, 1)]:
+    for tuning in [(0.0625, 1408, 512, 1), (1.0, 896, 512, 1)]:
         print(f"[dense-radii] spring={spring} H={half_height} {tuning}", flush=True)
         with Physics(ps) as g:
             g.set_tuning(*tuning)
@@ -674,7 +674,7 @@ def test_checkpoint_resume_is_bit_exact(gpu, split):
 
 
 TUNINGS = [(m, p, s, d) for d in (0, 1) for s in (512, 256) for p in (896, 1408)
-           for m in (1.0, 0.5, 0.25)]
+           for m in (1.0, 0.5, 0.25, 0.0625)]
 
 
 @pytest.mark.parametrize("scene", ["uniform", "dense", "clumps"])
