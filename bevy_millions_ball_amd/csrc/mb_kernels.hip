@@ -558,17 +558,36 @@ __device__ __forceinline__ void div3(float tx, float ty, float tz, float d, floa
     qz = tz / d;
 }
 
+// sqrtf for x >= 0.  On [2^-96, 2^96] the hardware square root followed by
+// the two one-ulp residual corrections of the compiler's own expansion is
+// sqrtf without its scaling steps and special-value checks (a GPU probe over
+// every float in that interval: 0 mismatches); other operands take sqrtf.
+__device__ __forceinline__ float sqrt_rn(float x)
+{
+    if (x >= 0x1p-96f && x <= 0x1p96f) {
+        const float s = __builtin_amdgcn_sqrtf(x);
+        const float sdn = __uint_as_float(__float_as_uint(s) - 1u);
+        const float sup = __uint_as_float(__float_as_uint(s) + 1u);
+        const float rdn = __builtin_fmaf(-sdn, s, x);
+        const float rup = __builtin_fmaf(-sup, s, x);
+        const float r = rdn <= 0.0f ? sdn : s;
+        return rup > 0.0f ? sup : r;
+    }
+    return sqrtf(x);
+}
+
 // solve_sphere_constraint (physics.wgsl:300-331) from the offset
 // t = A.position - B.position, d2 = dot(t, t) and edge = r_B + r_A.
 // `vel_mass()` yields the neighbour's velocity and mass; it is called only
-// for a contact with the rigid response.  Returns whether the spheres touch
-// (only then can the sphere's position change).
+// for a contact with the rigid response.  Returns whether the sphere's
+// position may have changed: at every contact with the rigid response, and
+// with the spring response only where the overlap exceeds r_agent r_A.
 template <bool SPRING, typename VelMass>
 __device__ __forceinline__ bool sphere_contact(const PhysConsts &c, Agent &a, float edge,
                                                float tx, float ty, float tz, float d2,
                                                VelMass vel_mass)
 {
-    const float dist = sqrtf(d2);
+    const float dist = sqrt_rn(d2);
     if (dist <= edge) {
         float nx, ny, nz;
         div3(tx, ty, tz, dist, nx, ny, nz);  // normalize(t): t / length(t)
@@ -615,7 +634,9 @@ __device__ __forceinline__ bool sphere_contact(const PhysConsts &c, Agent &a, fl
                 a.px = a.px + limit * nx;
                 a.py = a.py + limit * ny;
                 a.pz = a.pz + limit * nz;
+                return true;
             }
+            return false;
         }
         return true;
     }
@@ -1377,16 +1398,16 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
             const uint32_t lz = (uint32_t)__builtin_clz(near);
             const uint32_t t = end - 32u + lz;
             near ^= 0x80000000u >> lz;
-            bool touch;
+            bool moved;
             if (FROM_LDS) {
                 const uint32_t l = lds_of(t);
-                touch = collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
+                moved = collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
             } else {
                 const uint32_t bo = perm[slot_of(t)];
-                touch = collide_candidate<SPRING>(c, a, pos_i[bo],
+                moved = collide_candidate<SPRING>(c, a, pos_i[bo],
                                                   [&]() { return vel_mass_of(vfme_i[bo]); });
             }
-            if (touch) {  // only a contact moves the sphere
+            if (moved) {
                 const float ex = a.px - rx, ey = a.py - ry, ez = a.pz - rz;
                 if (!(ex * ex + ey * ey + ez * ez <= still)) {
                     next = t + 1u;  // moved beyond the margin: rescan after t
@@ -1503,8 +1524,8 @@ __device__ __forceinline__ void walk_dense(const PhysConsts &c, Agent &a, uint32
             mask[2] ^= (!h0 && !h1 && h2) ? bit : 0u;
             mask[3] ^= (!h0 && !h1 && !h2) ? bit : 0u;
             const uint32_t l = t + l0;
-            const bool touch = collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
-            if (touch) {  // only a contact moves the sphere
+            const bool moved = collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
+            if (moved) {
                 const float ex = a.px - rx, ey = a.py - ry, ez = a.pz - rz;
                 if (!(ex * ex + ey * ey + ez * ez <= still)) {
                     next = t + 1u;  // moved beyond the margin: rescan after t
