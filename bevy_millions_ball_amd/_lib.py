@@ -140,6 +140,8 @@ SIGNATURES = {
         C.c_int,
         [_P, _P, _P, _P, C.c_uint32, C.c_uint32, C.c_uint32, _P],
     ),
+    "mb_debug_check_rewrites": (
+        C.c_int, [C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "mb_spatial_hashing": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P]),
     "mb_physics_run": (
         C.c_int,

@@ -1001,6 +1001,23 @@ int mb_physics_autotune(mb_physics *p, uint32_t frames, mb_tuning *best, double 
     return MB_OK;
 }
 
+int mb_debug_check_rewrites(uint64_t div_samples, uint64_t *sqrt_mismatches,
+                            uint64_t *div_mismatches)
+{
+    if (!sqrt_mismatches || !div_mismatches) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    unsigned long long *bad = nullptr;
+    MB_HIP(hipMalloc(&bad, 2 * sizeof(unsigned long long)));
+    hipError_t e = hipMemset(bad, 0, 2 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = mb::launch_rewrite_check(div_samples, bad, nullptr);
+    unsigned long long h[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpy(h, bad, sizeof(h), hipMemcpyDeviceToHost);
+    (void)hipFree(bad);
+    if (e != hipSuccess) return fail_hip(e, "rewrite check");
+    *sqrt_mismatches = h[0];
+    *div_mismatches = h[1];
+    return MB_OK;
+}
+
 int64_t mb_physics_time(const mb_physics *p) { return p ? p->time : -1; }
 
 int mb_physics_is_even(const mb_physics *p)

@@ -119,6 +119,11 @@ struct StateCheck {
     uint32_t pad;
 };
 
+// Self-check of the contact solve's exact rewrites (sqrt_rn on [2^-96, 2^96]
+// against sqrtf, div3 against '/' on `samples` operand triples): adds the
+// mismatch counts to bad[0] and bad[1].
+hipError_t launch_rewrite_check(uint64_t samples, unsigned long long *bad, hipStream_t stream);
+
 // `out` holds 1 + kStateCheckParts entries: out[0] receives the result of
 // the N > 0 spheres, the others the per-block partials.  N = 0 launches
 // nothing (out[0] keeps what the host put there).

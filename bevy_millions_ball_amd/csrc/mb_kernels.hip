@@ -576,6 +576,66 @@ __device__ __forceinline__ float sqrt_rn(float x)
     return sqrtf(x);
 }
 
+// Self-check of the two exact rewrites above, for tests: bad[0] counts the
+// floats x in [2^-96, 2^96] where sqrt_rn(x) and sqrtf(x) differ in any bit
+// (all 1.6e9 of them), bad[1] the operand triples, of `samples`
+// pseudo-random ones, where div3 and '/' differ.  Divisors have exponents in
+// [-64, 63] (both sides of div3's range); numerators are +-0, +-d, or +-d u
+// with u in [2^-91, 2) (tiny ones included, and |t| a little above d, which
+// rounding in d = sqrt(dot(t, t)) can give).
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ float check_numerator(float d, uint64_t h)
+{
+    const uint32_t kind = (uint32_t)(h & 15u);
+    if (kind < 4u) return kind == 0u ? 0.0f : (kind == 1u ? -0.0f : (kind == 2u ? d : -d));
+    const uint32_t k = (uint32_t)((h >> 4) % 92u);                 // 0 .. 91
+    const float m = __uint_as_float(0x3F800000u | (uint32_t)((h >> 12) & 0x7FFFFFu));  // [1, 2)
+    const float u = m * __builtin_ldexpf(1.0f, -(int)k);            // [2^-91, 2)
+    const float t = d * u;
+    return ((h >> 40) & 1u) ? -t : t;
+}
+
+__global__ __launch_bounds__(256) void rewrite_check_kernel(uint64_t samples,
+                                                            unsigned long long *__restrict__ bad)
+{
+    const uint32_t tid = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t nt = gridDim.x * 256u;
+    unsigned long long bs = 0, bd = 0;
+    for (uint32_t b = 0x0F800000u + tid; b <= 0x6F800000u; b += nt) {
+        const float x = __uint_as_float(b);
+        bs += __float_as_uint(sqrt_rn(x)) != __float_as_uint(sqrtf(x)) ? 1u : 0u;
+    }
+    for (uint64_t i = tid; i < samples; i += nt) {
+        const uint64_t h0 = mix64(2u * i), h1 = mix64(2u * i + 1u);
+        const int e = (int)(h0 & 127u) - 64;                          // -64 .. 63
+        const float d = __builtin_ldexpf(
+            __uint_as_float(0x3F800000u | (uint32_t)((h0 >> 7) & 0x7FFFFFu)), e);
+        const float tx = check_numerator(d, h0 >> 30), ty = check_numerator(d, h1),
+                    tz = check_numerator(d, h1 >> 23);
+        float qx, qy, qz;
+        div3(tx, ty, tz, d, qx, qy, qz);
+        bd += (__float_as_uint(qx) != __float_as_uint(tx / d) ||
+               __float_as_uint(qy) != __float_as_uint(ty / d) ||
+               __float_as_uint(qz) != __float_as_uint(tz / d))
+                  ? 1u
+                  : 0u;
+    }
+    if (bs) atomicAdd(&bad[0], bs);
+    if (bd) atomicAdd(&bad[1], bd);
+}
+
+hipError_t launch_rewrite_check(uint64_t samples, unsigned long long *bad, hipStream_t stream)
+{
+    hipLaunchKernelGGL(rewrite_check_kernel, dim3(8192), dim3(256), 0, stream, samples, bad);
+    return hipGetLastError();
+}
+
 // solve_sphere_constraint (physics.wgsl:300-331) from the offset
 // t = A.position - B.position, d2 = dot(t, t) and edge = r_B + r_A.
 // `vel_mass()` yields the neighbour's velocity and mass; it is called only

@@ -360,6 +360,17 @@ def physics_run(settings: PhysicsSettings, init: bool, perm, start, close, key_o
         p(vfme_i), p(pos_o), p(vfme_o), int(number_of_agents), _stream_ptr(stream)))
 
 
+def check_exact_rewrites(div_samples: int = 1 << 28):
+    """Runs the library's self-check of the contact solve's exact rewrites on
+    the current device: (sqrt mismatches over every float in [2^-96, 2^96],
+    quotient mismatches over `div_samples` operand triples); both 0 on a
+    correct build."""
+    s = C.c_uint64(0)
+    d = C.c_uint64(0)
+    check(lib().mb_debug_check_rewrites(int(div_samples), C.byref(s), C.byref(d)))
+    return int(s.value), int(d.value)
+
+
 __all__ = [
     "PhysicsPlugin",
     "PhysicsSettings",
@@ -370,6 +381,7 @@ __all__ = [
     "default_settings_from_library",
     "radix_sort_workspace_size",
     "radix_sort_run",
+    "check_exact_rewrites",
     "spatial_hashing_run",
     "physics_run",
 ]

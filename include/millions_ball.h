@@ -256,6 +256,17 @@ int mb_radix_sort_static(const uint32_t *keys_in, const uint32_t *vals_in, uint3
                          uint32_t *vals_out, uint32_t number_of_keys, uint32_t pass_begin,
                          uint32_t pass_end, void *stream);
 
+/* No counterpart in the reference; a self-check for tests.  Runs on the
+ * current device, synchronously, the check of the contact solve's exact
+ * rewrites: *sqrt_mismatches = the floats x in [2^-96, 2^96] (every one of
+ * them) where its short square root differs from sqrtf in any bit;
+ * *div_mismatches = the operand triples, of div_samples pseudo-random ones
+ * (divisors both inside and outside the shared-reciprocal range, numerators
+ * +-0, +-d, tiny and a little above d), where its shared-reciprocal quotients
+ * differ from '/'.  Both are 0 on a correct build. */
+int mb_debug_check_rewrites(uint64_t div_samples, uint64_t *sqrt_mismatches,
+                            uint64_t *div_mismatches);
+
 /* spatial_hashing::run(.., number_of_keys, read_from_even)
  * (spatial_hashing.rs:191-243): start[0..zero_count) = MB_INVALID_INDEX, then
  * start[k] / close[k] for the first / last sorted index of every key k.  Keys

@@ -136,6 +136,8 @@ extern "C" {
     pub fn mb_radix_sort_static(keys_in: *const u32, vals_in: *const u32, keys_out: *mut u32,
                                 vals_out: *mut u32, number_of_keys: u32, pass_begin: u32,
                                 pass_end: u32, stream: *mut c_void) -> c_int;
+    pub fn mb_debug_check_rewrites(div_samples: u64, sqrt_mismatches: *mut u64,
+                                   div_mismatches: *mut u64) -> c_int;
     pub fn mb_spatial_hashing(sorted_keys: *const u32, start: *mut u32, close: *mut u32,
                               number_of_keys: u32, zero_count: u32, stream: *mut c_void) -> c_int;
     pub fn mb_physics_run(settings: *const mb_settings, init: c_int, perm: *const u32,

@@ -592,6 +592,20 @@ def test_edge_cases_tiny_worlds(gpu, oracle):
             compare_frames(g, r, 8)
 
 
+def test_exact_rewrites_against_ieee(gpu):
+    """The contact solve's short square root equals sqrtf on every float in
+    [2^-96, 2^96], and its shared-reciprocal quotients equal '/' on 2^28
+    operand triples (divisors inside and outside the rewrite's range; zero,
+    signed-zero, +-d, tiny and slightly-above-d numerators), bit for bit, on
+    the device itself."""
+    from bevy_millions_ball_amd.physics import check_exact_rewrites
+
+    print("[rewrites] sqrt exhaustive, div 2^28", flush=True)
+    sqrt_bad, div_bad = check_exact_rewrites(1 << 28)
+    assert sqrt_bad == 0, f"{sqrt_bad} square roots differ from sqrtf"
+    assert div_bad == 0, f"{div_bad} quotient triples differ from '/'"
+
+
 def test_exact_rewrite_fallback_paths(gpu, oracle):
     """The kernels' exact rewrites next to their general code, in one wave:
     contacts whose offset has a component below 2^-60 (general division),
