@@ -28,6 +28,7 @@
 #include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -707,7 +708,6 @@ int mb_server_submit(mb_server *s, const mb_settings *settings, const float *pos
                      const uint32_t *vfme, uint32_t steps, int keep_debug, uint64_t *id)
 {
     if (!s || !settings || !pos || !vfme || !id) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
-    std::vector<uint32_t> rows;  // spheres per row of the submitted state
     int rc = validate_scene(*settings);
     if (rc != MB_OK) return rc;
     const uint32_t n = settings->max_number_of_agents;
@@ -715,52 +715,119 @@ int mb_server_submit(mb_server *s, const mb_settings *settings, const float *pos
     const SceneLayout layout = scene_layout(n, g);
     if (layout.bytes > s->config.memory_cap_bytes)
         return fail(MB_ERR_INVALID_ARGUMENT, "scene needs more device memory than the cap");
-    for (size_t i = 0; i < (size_t)n * 4u; ++i) {
-        if (!std::isfinite(pos[i]))
-            return fail(MB_ERR_INVALID_ARGUMENT, "non-finite position_with_scale at float " + std::to_string(i));
-        if (!finite_half(vfme[i] & 0xFFFFu) || !finite_half(vfme[i] >> 16))
-            return fail(MB_ERR_INVALID_ARGUMENT, "non-finite half in packed_vfme at word " + std::to_string(i));
-    }
-    {
+    // Validation, the row counts and the copy into pinned memory, split over
+    // up to 8 host threads for large scenes (the caller's thread also ticks
+    // the server: a 1M-sphere submit took 5 ms on one thread).  The first
+    // offending index is reported, as one pass in order would find it.
+    const mb::PhysConsts c = mb::compile_consts(*settings);
+    const size_t io_bytes = (size_t)n * (32u + (keep_debug ? 8u : 0u));
+    size_t io_cls = 0;
+    void *io = s->pinned.get(io_bytes, io_cls);
+    if (!io) return fail(MB_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
+    struct Part {
+        size_t bad = SIZE_MAX;      // first float index with a non-finite value
+        bool bad_pos = false;       // ... in position_with_scale (else packed_vfme)
+        size_t outside = SIZE_MAX;  // first sphere outside the grid
+        float r_min = INFINITY, r_max = -INFINITY;
+        std::vector<uint32_t> rows;
+    };
+    const unsigned parts = (unsigned)std::min<size_t>(8u, std::max<size_t>(1u, n / 65536u));
+    std::vector<Part> part(parts);
+    auto work = [&](unsigned k) {
+        Part &q = part[k];
+        const size_t lo = (size_t)n * k / parts, hi = (size_t)n * (k + 1u) / parts;
+        std::memcpy(static_cast<char *>(io) + lo * 16u, pos + lo * 4u, (hi - lo) * 16u);
+        std::memcpy(static_cast<char *>(io) + (size_t)n * 16u + lo * 16u, vfme + lo * 4u,
+                    (hi - lo) * 16u);
+        for (size_t i = lo * 4u; i < hi * 4u; ++i) {
+            if (!std::isfinite(pos[i])) {
+                q.bad = i;
+                q.bad_pos = true;
+                return;
+            }
+            if (!finite_half(vfme[i] & 0xFFFFu) || !finite_half(vfme[i] >> 16)) {
+                q.bad = i;
+                return;
+            }
+        }
         // The batched frame runs the frame path only, whose buckets cover the
         // grid: every sphere's cell must lie in it, and the radii must keep
         // wall-clamped positions there (the single-scene handle runs other
         // states through the stage kernels).
-        const mb::PhysConsts c = mb::compile_consts(*settings);
-        float r_min = pos[3], r_max = pos[3];
-        rows.assign(g, 0u);
-        for (uint32_t i = 0; i < n; ++i) {
-            const float *q = pos + 4u * (size_t)i;
-            const uint32_t cz = mb::host_cell(c, q[2]);
-            if (mb::host_cell(c, q[0]) >= g || cz >= g)
-                return fail(MB_ERR_INVALID_ARGUMENT,
-                            "sphere " + std::to_string(i) + " lies outside the grid (serving needs every cell in it)");
-            rows[cz] += 1u;
-            r_min = std::min(r_min, q[3]);
-            r_max = std::max(r_max, q[3]);
+        q.rows.assign(g, 0u);
+        for (size_t i = lo; i < hi; ++i) {
+            const float *v = pos + 4u * i;
+            const uint64_t cx = mb::host_cell(c, v[0]), cz = mb::host_cell(c, v[2]);
+            if (cx >= g || cz >= g) {
+                q.outside = i;
+                return;
+            }
+            q.rows[cz] += 1u;
+            q.r_min = std::min(q.r_min, v[3]);
+            q.r_max = std::max(q.r_max, v[3]);
         }
-        if (r_min < 0.0f)
-            return fail(MB_ERR_INVALID_ARGUMENT,
-                        "a negative radius (position_with_scale.w): radii must be >= 0");
-        if (mb::loop_cell_bound(c, r_min, r_max) >= g)
-            return fail(MB_ERR_INVALID_ARGUMENT,
-                        "the radii let wall-clamped positions leave the grid (serving needs every cell in it)");
+    };
+    {
+        std::vector<std::thread> pool;
+        unsigned started = 1;
+        try {
+            for (; started < parts; ++started) pool.emplace_back(work, started);
+        } catch (...) {
+            // no more threads: the caller's thread takes the rest
+        }
+        work(0);
+        for (unsigned k = started; k < parts; ++k) work(k);
+        for (std::thread &th : pool) th.join();
+    }
+    size_t bad = SIZE_MAX, outside = SIZE_MAX;
+    bool bad_pos = false;
+    float r_min = INFINITY, r_max = -INFINITY;
+    for (const Part &q : part) {
+        if (q.bad < bad) {
+            bad = q.bad;
+            bad_pos = q.bad_pos;
+        }
+        outside = std::min(outside, q.outside);
+        r_min = std::min(r_min, q.r_min);
+        r_max = std::max(r_max, q.r_max);
+    }
+    std::string why;
+    if (bad != SIZE_MAX)
+        why = bad_pos ? "non-finite position_with_scale at float " + std::to_string(bad)
+                      : "non-finite half in packed_vfme at word " + std::to_string(bad);
+    else if (outside != SIZE_MAX)
+        why = "sphere " + std::to_string(outside) +
+              " lies outside the grid (serving needs every cell in it)";
+    else if (r_min < 0.0f)
+        why = "a negative radius (position_with_scale.w): radii must be >= 0";
+    else if (mb::loop_cell_bound(c, r_min, r_max) >= g)
+        why = "the radii let wall-clamped positions leave the grid (serving needs every cell in it)";
+    if (!why.empty()) {
+        s->pinned.put(io, io_cls);
+        return fail(MB_ERR_INVALID_ARGUMENT, why);
+    }
+    uint32_t row_peak = 0;
+    for (uint32_t row = 0; row < g; ++row) {
+        uint32_t sum = 0;
+        for (const Part &q : part) sum += q.rows[row];
+        row_peak = std::max(row_peak, sum);
     }
     std::unique_ptr<Request> r(new (std::nothrow) Request());
-    if (!r) return fail(MB_ERR_OUT_OF_MEMORY, "host allocation failed");
+    if (!r) {
+        s->pinned.put(io, io_cls);
+        return fail(MB_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
     r->id = s->next_id++;
     r->settings = *settings;
-    r->consts = mb::compile_consts(*settings);
+    r->consts = c;
     r->n = n;
     r->g = g;
-    r->row_peak = rows.empty() ? 0u : *std::max_element(rows.begin(), rows.end());
+    r->row_peak = row_peak;
     r->steps = steps;
     r->keep_debug = keep_debug != 0;
     r->layout = layout;
-    r->io = s->pinned.get(r->out_bytes(), r->io_cls);
-    if (!r->io) return fail(MB_ERR_OUT_OF_MEMORY, "pinned host allocation failed");
-    std::memcpy(r->io, pos, (size_t)n * 16);
-    std::memcpy(static_cast<char *>(r->io) + (size_t)n * 16, vfme, (size_t)n * 16);
+    r->io = io;
+    r->io_cls = io_cls;
     r->submit_s = now_s();
     *id = r->id;
     s->queue.push_back(r.get());
