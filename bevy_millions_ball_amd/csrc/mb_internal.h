@@ -135,6 +135,7 @@ hipError_t launch_state_check(const PhysConsts &c, const float4 *pos, const uint
 constexpr uint32_t kMaxRows = 4096;     // G <= 4096 (G^2 <= 2^24)
 constexpr uint32_t kRowCap = 4096;      // spheres of one row sorted in LDS
 constexpr uint32_t kRowCapNarrow = 2048; // the same for the batched sort of sparse rows (256 threads)
+constexpr uint32_t kRowCapWide = 8192;   // the same for grids whose mean row exceeds 3072 spheres
 constexpr uint32_t kSortThreads = 512;  // threads of a row sort block (256 or 512)
 constexpr uint32_t kStagePool = 896;    // staged slots of a block's three neighbour-row windows
 constexpr uint32_t kStagePoolBig = 1408; // the same for dense scenes (fewer blocks per CU)

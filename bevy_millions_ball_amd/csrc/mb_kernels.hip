@@ -1881,6 +1881,14 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time
                            (row_sort_lds<256, kRowCapNarrow>(G)), stream, b.rb, set,
                            time == 1 ? 1u : 0u, b.scratch_pairs, b.scratch_idx, G, n, b.perm, b.off,
                            b.keys_out);
+    else if ((n + G - 1u) / G > 3072u)
+        // rows of ~4096 spheres and more (2^24 spheres on G = 4096): about
+        // half would exceed kRowCap and be sorted in global memory; 48 KB of
+        // LDS per block at G = 4096
+        hipLaunchKernelGGL((row_sort_kernel<kSortThreads, kRowCapWide>), dim3(G),
+                           dim3(kSortThreads), (row_sort_lds<kSortThreads, kRowCapWide>(G)), stream,
+                           b.rb, set, time == 1 ? 1u : 0u, b.scratch_pairs, b.scratch_idx, G, n,
+                           b.perm, b.off, b.keys_out);
     else
         hipLaunchKernelGGL((row_sort_kernel<kSortThreads, kRowCap>), dim3(G), dim3(kSortThreads),
                            row_sort_lds(G), stream, b.rb, set, time == 1 ? 1u : 0u, b.scratch_pairs,

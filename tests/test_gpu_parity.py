@@ -285,8 +285,9 @@ def test_dense_scan_mixed_radii_and_heights(gpu, oracle, spring, half_height):
 
 def test_largest_grid_bit_exact(gpu, oracle):
     """The largest grid the keys allow (G = 4096, G^2 = 2^24 cells) with
-    N = G^2 spheres: ~4096 spheres per row, so about half the rows exceed the
-    in-LDS row size and take the global-memory row sort."""
+    N = G^2 spheres: ~4096 spheres per row, about half of them above the
+    common row sort's 4096 in-LDS places, which makes the mean row (4096 >
+    3072) pick the variant with 8192 places."""
     cfg = replace(CONFIGS["uniform_1m"], number_of_agents=1 << 24, grids=4096)
     print("[max] G=4096 n=16777216", flush=True)
     g, r = make_pair(oracle, cfg, seed=3)
