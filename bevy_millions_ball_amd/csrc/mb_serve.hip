@@ -730,6 +730,7 @@ int mb_server_submit(mb_server *s, const mb_settings *settings, const float *pos
         size_t outside = SIZE_MAX;  // first sphere outside the grid
         float r_min = INFINITY, r_max = -INFINITY;
         std::vector<uint32_t> rows;
+        bool no_memory = false;     // the row counts could not be allocated
     };
     const unsigned parts = (unsigned)std::min<size_t>(8u, std::max<size_t>(1u, n / 65536u));
     std::vector<Part> part(parts);
@@ -754,7 +755,12 @@ int mb_server_submit(mb_server *s, const mb_settings *settings, const float *pos
         // grid: every sphere's cell must lie in it, and the radii must keep
         // wall-clamped positions there (the single-scene handle runs other
         // states through the stage kernels).
-        q.rows.assign(g, 0u);
+        try {
+            q.rows.assign(g, 0u);
+        } catch (...) {  // nothing may leave a worker thread
+            q.no_memory = true;
+            return;
+        }
         for (size_t i = lo; i < hi; ++i) {
             const float *v = pos + 4u * i;
             const uint64_t cx = mb::host_cell(c, v[0]), cz = mb::host_cell(c, v[2]);
@@ -780,9 +786,10 @@ int mb_server_submit(mb_server *s, const mb_settings *settings, const float *pos
         for (std::thread &th : pool) th.join();
     }
     size_t bad = SIZE_MAX, outside = SIZE_MAX;
-    bool bad_pos = false;
+    bool bad_pos = false, no_memory = false;
     float r_min = INFINITY, r_max = -INFINITY;
     for (const Part &q : part) {
+        no_memory = no_memory || q.no_memory;
         if (q.bad < bad) {
             bad = q.bad;
             bad_pos = q.bad_pos;
@@ -790,6 +797,10 @@ int mb_server_submit(mb_server *s, const mb_settings *settings, const float *pos
         outside = std::min(outside, q.outside);
         r_min = std::min(r_min, q.r_min);
         r_max = std::max(r_max, q.r_max);
+    }
+    if (no_memory && bad == SIZE_MAX) {
+        s->pinned.put(io, io_cls);
+        return fail(MB_ERR_OUT_OF_MEMORY, "host allocation failed");
     }
     std::string why;
     if (bad != SIZE_MAX)
