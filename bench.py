@@ -261,10 +261,12 @@ def copy_calibration(launches: int = 5) -> int:
     return 0
 
 
-def other_config(name: str, seed: int, steps: int) -> dict:
+def other_config(name: str, seed: int, steps: int, cpu_seconds: float = 0.0,
+                 cpu_threads: int = 1) -> dict:
     """Throughput and per-kernel device time of another workload (the
     parity-test configurations), `steps` LOOP frames after INIT + 10 and the
-    autotuner's trials."""
+    autotuner's trials; with cpu_seconds > 0 also the oracle on the host for
+    a sample of about that long (and a one-thread sample half as long)."""
     import torch
 
     from bevy_millions_ball_amd import CONFIGS, Physics, generate
@@ -290,6 +292,7 @@ def other_config(name: str, seed: int, steps: int) -> dict:
         stage_ms = phys.profile_read().mean(axis=0)
         phys.set_profiling(False)
     achieved = physics_bytes_per_sphere(n, cells) * n / (stage_ms[1] * 1e-3) / 1e9
+    cpu = cpu_baseline(cfg, pos, vf, cpu_seconds, cpu_threads) if cpu_seconds > 0 else None
     return {
         "spheres": n, "grids": cfg.grids, "spring": cfg.enable_spring, "layout": cfg.layout,
         "time_delta": cfg.time_delta, "steps": steps,
@@ -298,6 +301,7 @@ def other_config(name: str, seed: int, steps: int) -> dict:
         "value": n * steps / el, "unit": "sphere-steps/s", "ms_per_step": el * 1e3 / steps,
         "stage_ms": {"row_sort": float(stage_ms[0]), "collide_integrate": float(stage_ms[1])},
         "collide_roofline_frac": achieved / HBM_PEAK_GBS,
+        "cpu_baseline": cpu,
     }
 
 
@@ -450,13 +454,16 @@ def main() -> int:
             "bytes_per_launch": phys_bytes,
         },
     }
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if not args.no_other_configs and args.config == "uniform_1m":
-        line["other_configs"] = {name: other_config(name, args.seed, steps) for name, steps in (
-            ("plain_65k", 500), ("spring_clumps_1m", 100), ("uniform_4m", 100))}
+        # each other workload's CPU sample: 40 % of the headline's budget
+        other_cpu = 0.0 if args.no_cpu_baseline else 0.4 * args.cpu_seconds
+        line["other_configs"] = {
+            name: other_config(name, args.seed, steps, other_cpu, threads)
+            for name, steps in (("plain_65k", 500), ("spring_clumps_1m", 100), ("uniform_4m", 100))}
     if not args.no_serving:
         line["serving"] = serving_bench(args.serving_requests, args.poisson_requests, args.seed)
     if not args.no_cpu_baseline:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         line["cpu_baseline"] = cpu_baseline(cfg, pos, vf, args.cpu_seconds, threads)
     print(json.dumps(line), flush=True)
     return 0
