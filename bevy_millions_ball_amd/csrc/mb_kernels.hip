@@ -1568,27 +1568,31 @@ __device__ __forceinline__ void walk_dense(const PhysConsts &c, Agent &a, uint32
             mask[g] = m;
             b = e;
         }
+        // pool index of bit 0 of each chunk
+        uint32_t pb[kDenseGroup];
+#pragma unroll
+        for (int g = 0; g < kDenseGroup; ++g) pb[g] = first[g] + lb[g];
         uint32_t next = b;
         for (;;) {
             // the first chunk with a near candidate left, chosen without a branch
             const bool h0 = mask[0] != 0u, h1 = mask[1] != 0u, h2 = mask[2] != 0u;
             const uint32_t m = h0 ? mask[0] : (h1 ? mask[1] : (h2 ? mask[2] : mask[3]));
             if (m == 0u) break;
-            const uint32_t f = h0 ? first[0] : (h1 ? first[1] : (h2 ? first[2] : first[3]));
-            const uint32_t l0 = h0 ? lb[0] : (h1 ? lb[1] : (h2 ? lb[2] : lb[3]));
+            const uint32_t p0 = h0 ? pb[0] : (h1 ? pb[1] : (h2 ? pb[2] : pb[3]));
             const uint32_t i = (uint32_t)__builtin_ctz(m);
-            const uint32_t t = f + i;
-            const uint32_t bit = 1u << i;
-            mask[0] ^= h0 ? bit : 0u;
-            mask[1] ^= (!h0 && h1) ? bit : 0u;
-            mask[2] ^= (!h0 && !h1 && h2) ? bit : 0u;
-            mask[3] ^= (!h0 && !h1 && !h2) ? bit : 0u;
-            const uint32_t l = t + l0;
+            const uint32_t rest = m & (m - 1u);  // the chunk without this candidate
+            mask[0] = h0 ? rest : mask[0];
+            mask[1] = (!h0 && h1) ? rest : mask[1];
+            mask[2] = (!h0 && !h1 && h2) ? rest : mask[2];
+            mask[3] = (!h0 && !h1 && !h2) ? rest : mask[3];
+            const uint32_t l = p0 + i;
             const bool moved = collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
             if (moved) {
                 const float ex = a.px - rx, ey = a.py - ry, ez = a.pz - rz;
                 if (!(ex * ex + ey * ey + ez * ez <= still)) {
-                    next = t + 1u;  // moved beyond the margin: rescan after t
+                    // moved beyond the margin: rescan after this candidate
+                    const uint32_t f = h0 ? first[0] : (h1 ? first[1] : (h2 ? first[2] : first[3]));
+                    next = f + i + 1u;
                     break;
                 }
             }
