@@ -1349,13 +1349,14 @@ __device__ __forceinline__ bool collide_candidate(const PhysConsts &c, Agent &a,
 // The sphere's state changes only at a contact, and a contact needs
 // |A - B| <= r_A + r_B.  Each chunk of up to 32 candidates is therefore first
 // scanned against the position `ref` the sphere had at the start of the
-// chunk, with the reach widened by a margin M = r_A: a candidate outside
-// that reach cannot be in contact while |A - ref| <= M.  (A wider margin
-// passes more distant candidates to the exact test, a narrower one restarts
-// the scan more often; r_A measured fastest on uniform scenes.)  Only the candidates
-// inside it (a bitmask) go through solve_sphere_constraint, in order; if a
-// contact moves the sphere by more than M the scan restarts right after that
-// candidate from the new position.  The sequence of state changes is the
+// chunk, with the reach widened by a margin M = scan_margin r_A: a candidate
+// outside that reach cannot be in contact while |A - ref| <= M.  (A wider
+// margin passes more distant candidates to the exact test, a narrower one
+// restarts the scan more often; M = r_A measured fastest on uniform scenes,
+// r_A / 16 on spring scenes, whose contacts move a sphere only at deep
+// overlaps.)  Only the candidates inside it (a bitmask) go through
+// solve_sphere_constraint, in order; if a contact moves the sphere by more
+// than M the scan restarts right after that candidate from the new position.  The sequence of state changes is the
 // shader's; only the wasted work on distant candidates is gone, and with it
 // most of the wave divergence of the contact path.
 //
