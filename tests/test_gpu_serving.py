@@ -239,6 +239,32 @@ def test_submit_rejects_states_outside_the_grid(gpu):
         srv.collect(rid)
 
 
+def test_submit_reports_the_first_offending_sphere(gpu):
+    """A large scene is checked by several host threads, each over a range of
+    spheres; whichever thread finds a problem, the message names the first
+    offending index in the order of one pass: non-finite values before cells
+    outside the grid, and the smallest index of each kind."""
+    n = 300_000                                  # four parts of 75,000 spheres
+    st, pos, vf, _ = scene(n, 2, 13)
+    two_out = pos.copy()
+    two_out[250_000, 0] = 1e6                    # fourth part
+    two_out[120_000, 2] = 1e6                    # second part: the one to name
+    nan_late = two_out.copy()
+    nan_late[260_000, 1] = np.nan                # a non-finite float outranks both
+    bad_half = vf.copy()
+    bad_half[200_000, 1] = 0x7C00                # +inf in an f16 half, third part
+    with Server() as srv:
+        with pytest.raises(Exception, match="sphere 120000 lies outside"):
+            srv.submit(st, two_out, vf, 2)
+        with pytest.raises(Exception, match=f"position_with_scale at float {260_000 * 4 + 1}"):
+            srv.submit(st, nan_late, vf, 2)
+        with pytest.raises(Exception, match=f"packed_vfme at word {200_000 * 4 + 1}"):
+            srv.submit(st, pos, bad_half, 2)
+        rid = srv.submit(st, pos, vf, 1)
+        srv.drain()
+        srv.collect(rid)
+
+
 def test_bench_request_mix_subset(gpu, oracle):
     """The first 10 requests of bench.py's own serving batch (poisson_requests
     with its seed 0: N log-uniform in [4096, 1,048,576], G = next pow2 of
