@@ -266,6 +266,10 @@ struct BatchPlan {
     uint32_t blk_plain_count;
     const uint2 *blk_spring;  // blocks of 256 sorted slots, spring on
     uint32_t blk_spring_count;
+    const uint2 *blk_plain_dense;   // the same for scenes with crowded cells (dense collide)
+    uint32_t blk_plain_dense_count;
+    const uint2 *blk_spring_dense;
+    uint32_t blk_spring_dense_count;
     uint32_t max_grids;       // largest G among the scenes (row sort LDS)
     uint32_t max_row_mean;    // largest ceil(N / G) among the scenes (row sort width)
     uint32_t max_row_peak;    // largest row of any scene's submitted state (row sort width)

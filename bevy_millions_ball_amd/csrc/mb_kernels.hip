@@ -1978,13 +1978,17 @@ __global__ __launch_bounds__(NT) void batch_row_sort_kernel(const SceneDesc *__r
                       gptr(S.scratch_idx), G, S.n, gptr(S.perm), off, gptr(S.keys), m.y);
 }
 
-template <bool SPRING>
+// DENSE: the scenes whose submitted state crowds its cells, on the dense
+// collide variant with the big staging pool.
+template <bool SPRING, bool DENSE>
 __global__ __launch_bounds__(256) void batch_physics_step_kernel(
     const SceneDesc *__restrict__ scenes, const uint2 *__restrict__ map, uint32_t k,
     const uint32_t *__restrict__ counter)
 {
-    __shared__ float4 s_pos[kStagePool];
-    __shared__ uint2 s_vm[kStagePool];
+    constexpr uint32_t POOL = DENSE ? kStagePoolBig : kStagePool;
+    __shared__ float4 s_pos[POOL];
+    __shared__ uint2 s_vm[POOL];
+    __shared__ uint32_t s_slab[DENSE ? kSlabs * slab_words(POOL) : 1u];
     const uint2 m = map[xcd_block(blockIdx.x, gridDim.x)];
     const SceneDesc &S = scenes[m.x];
     const int32_t time = scene_time(S, plan_frame(k, counter));
@@ -1996,9 +2000,9 @@ __global__ __launch_bounds__(256) void batch_physics_step_kernel(
         return;
     }
     const uint32_t p = (uint32_t)time & 1u;
-    physics_step_body<SPRING, kStagePool>(S.c, gptr(S.perm), gptr(S.off), gptr(S.keys), gptr(S.pos[p]),
-                              gptr(S.vfme[p]), gptr(S.pos[p ^ 1u]), gptr(S.vfme[p ^ 1u]), rb,
-                              p ^ 1u, S.n, m.y, s_pos, s_vm, nullptr);
+    physics_step_body<SPRING, POOL, false, DENSE>(
+        S.c, gptr(S.perm), gptr(S.off), gptr(S.keys), gptr(S.pos[p]), gptr(S.vfme[p]),
+        gptr(S.pos[p ^ 1u]), gptr(S.vfme[p ^ 1u]), rb, p ^ 1u, S.n, m.y, s_pos, s_vm, s_slab);
 }
 
 __global__ __launch_bounds__(64) void batch_advance_kernel(uint32_t *__restrict__ counter)
@@ -2047,11 +2051,17 @@ hipError_t launch_batch_frame(const BatchPlan &p, uint32_t k, bool from_counter,
                                p.row_map, k, counter);
     }
     if (p.blk_plain_count)
-        hipLaunchKernelGGL(batch_physics_step_kernel<false>, dim3(p.blk_plain_count), dim3(256), 0,
-                           stream, p.scenes, p.blk_plain, k, counter);
+        hipLaunchKernelGGL((batch_physics_step_kernel<false, false>), dim3(p.blk_plain_count),
+                           dim3(256), 0, stream, p.scenes, p.blk_plain, k, counter);
     if (p.blk_spring_count)
-        hipLaunchKernelGGL(batch_physics_step_kernel<true>, dim3(p.blk_spring_count), dim3(256), 0,
-                           stream, p.scenes, p.blk_spring, k, counter);
+        hipLaunchKernelGGL((batch_physics_step_kernel<true, false>), dim3(p.blk_spring_count),
+                           dim3(256), 0, stream, p.scenes, p.blk_spring, k, counter);
+    if (p.blk_plain_dense_count)
+        hipLaunchKernelGGL((batch_physics_step_kernel<false, true>), dim3(p.blk_plain_dense_count),
+                           dim3(256), 0, stream, p.scenes, p.blk_plain_dense, k, counter);
+    if (p.blk_spring_dense_count)
+        hipLaunchKernelGGL((batch_physics_step_kernel<true, true>), dim3(p.blk_spring_dense_count),
+                           dim3(256), 0, stream, p.scenes, p.blk_spring_dense, k, counter);
     if (from_counter)
         hipLaunchKernelGGL(batch_advance_kernel, dim3(1), dim3(64), 0, stream, p.frame_counter);
     return hipGetLastError();

@@ -213,6 +213,7 @@ struct Request {
     mb::PhysConsts consts{};
     uint32_t n = 0, g = 0, steps = 0;
     uint32_t row_peak = 0;      // spheres in the fullest row of the submitted state
+    bool dense = false;         // crowded cells: the dense collide variant
     bool keep_debug = false;
     int state = MB_REQUEST_QUEUED;
     SceneLayout layout{};
@@ -471,7 +472,7 @@ int upload_plan(mb_server *s)
 {
     const uint32_t count = (uint32_t)s->running.size();
     std::vector<mb::SceneDesc> descs;
-    std::vector<uint2> rows, plain, spring;
+    std::vector<uint2> rows, plain, spring, plain_dense, spring_dense;
     descs.reserve(count);
     uint32_t max_g = 1, max_rm = 1, max_peak = 0;
     for (uint32_t i = 0; i < count; ++i) {
@@ -481,13 +482,15 @@ int upload_plan(mb_server *s)
         max_rm = std::max(max_rm, (r->n + r->g - 1u) / r->g);
         max_peak = std::max(max_peak, r->row_peak);
         for (uint32_t g = 0; g < r->g; ++g) rows.push_back(make_uint2(i, g));
-        auto &blk = r->consts.enable_spring ? spring : plain;
+        auto &blk = r->consts.enable_spring ? (r->dense ? spring_dense : spring)
+                                            : (r->dense ? plain_dense : plain);
         for (uint32_t b = 0; b < (r->n + 255u) / 256u; ++b) blk.push_back(make_uint2(i, b));
     }
     // staged plan: frame counter (one aligned slot) | scene table | work lists
     const size_t head = kAlign;
     const size_t desc_bytes = align_up(descs.size() * sizeof(mb::SceneDesc));
-    const size_t map_count = rows.size() + plain.size() + spring.size();
+    const size_t map_count =
+        rows.size() + plain.size() + spring.size() + plain_dense.size() + spring_dense.size();
     const size_t need = head + desc_bytes + map_count * sizeof(uint2);
     if (s->plan_cap < need) {
         // grow geometrically; frames in flight may still read the old buffer
@@ -513,7 +516,7 @@ int upload_plan(mb_server *s)
     std::memcpy(st + head, descs.data(), descs.size() * sizeof(mb::SceneDesc));
     uint2 *m = reinterpret_cast<uint2 *>(st + head + desc_bytes);
     size_t at = 0;
-    for (auto *v : {&rows, &plain, &spring}) {
+    for (auto *v : {&rows, &plain, &spring, &plain_dense, &spring_dense}) {
         std::memcpy(m + at, v->data(), v->size() * sizeof(uint2));
         at += v->size();
     }
@@ -536,6 +539,10 @@ int upload_plan(mb_server *s)
     p.blk_plain_count = (uint32_t)plain.size();
     p.blk_spring = p.blk_plain + plain.size();
     p.blk_spring_count = (uint32_t)spring.size();
+    p.blk_plain_dense = p.blk_spring + spring.size();
+    p.blk_plain_dense_count = (uint32_t)plain_dense.size();
+    p.blk_spring_dense = p.blk_plain_dense + plain_dense.size();
+    p.blk_spring_dense_count = (uint32_t)spring_dense.size();
     p.max_grids = max_g;
     p.max_row_mean = max_rm;
     p.max_row_peak = max_peak;
@@ -823,6 +830,34 @@ int mb_server_submit(mb_server *s, const mb_settings *settings, const float *pos
         for (const Part &q : part) sum += q.rows[row];
         row_peak = std::max(row_peak, sum);
     }
+    // Crowding, from up to 16384 spheres taken at a fixed stride: the mean,
+    // over the sampled spheres, of the spheres per cell in the 8 x 8-cell
+    // block around each (its sampled count scaled to all spheres).  Four or
+    // more, about 36 candidates a sphere, takes the dense collide variant.
+    bool dense = false;
+    {
+        const uint32_t sb = (g + 7u) / 8u;  // blocks of 8 x 8 cells a side
+        const uint32_t stride = std::max<uint32_t>(1u, n / 16384u);
+        std::vector<uint32_t> blocks;
+        try {
+            blocks.assign((size_t)sb * sb, 0u);
+        } catch (...) {
+            s->pinned.put(io, io_cls);
+            return fail(MB_ERR_OUT_OF_MEMORY, "host allocation failed");
+        }
+        std::vector<uint32_t> at;
+        at.reserve(n / stride + 1u);
+        for (uint32_t i = 0; i < n; i += stride) {
+            const float *v = pos + 4u * (size_t)i;
+            const uint32_t b = (uint32_t)(mb::host_cell(c, v[2]) / 8u) * sb +
+                               (uint32_t)(mb::host_cell(c, v[0]) / 8u);
+            blocks[b] += 1u;
+            at.push_back(b);
+        }
+        double seen = 0.0;
+        for (uint32_t b : at) seen += (double)blocks[b] * stride / 64.0;
+        dense = !at.empty() && seen / (double)at.size() >= 4.0;
+    }
     std::unique_ptr<Request> r(new (std::nothrow) Request());
     if (!r) {
         s->pinned.put(io, io_cls);
@@ -831,9 +866,13 @@ int mb_server_submit(mb_server *s, const mb_settings *settings, const float *pos
     r->id = s->next_id++;
     r->settings = *settings;
     r->consts = c;
+    // spring contacts move a sphere only at deep overlaps, so a narrow scan
+    // margin (few rescans) suits them; results do not depend on it
+    r->consts.scan_margin = c.enable_spring ? 0.0625f : 1.0f;
     r->n = n;
     r->g = g;
     r->row_peak = row_peak;
+    r->dense = dense;
     r->steps = steps;
     r->keep_debug = keep_debug != 0;
     r->layout = layout;

@@ -25,7 +25,8 @@ def scene(n, steps, seed, grids=None, **kw):
 
 # sizes, grids and modes that exercise the batched kernels' scene lookups:
 # G^2 > N, G^2 == N, spring on, a single sphere, INIT only, a scene spanning
-# several partition tiles and collide blocks with a partial last block
+# several partition tiles and collide blocks with a partial last block, and
+# crowded scenes of both modes (the dense collide variant)
 MIX = [
     scene(4096, 5, 1),
     scene(5000, 3, 2),                      # G = 128, G^2 > N
@@ -33,6 +34,7 @@ MIX = [
     scene(65536, 4, 4, grids=256),          # G^2 == N
     scene(30000, 6, 5, grids=128, enable_spring=True, k_agent=500.0, c_agent=5.0,
           time_delta=1.0 / 120.0, layout="clumps", clumps=4, clump_sigma_cells=6.0),
+    scene(16000, 5, 15, grids=128, layout="clumps", clumps=3, clump_sigma_cells=5.0),  # crowded, rigid
     scene(1, 2, 6, grids=2),                # one sphere, G^2 > N
     scene(70001, 0, 7),                     # INIT only, G = 512
     scene(9999, 9, 8, grids=100),           # G not a power of two
