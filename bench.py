@@ -297,7 +297,8 @@ def other_config(name: str, seed: int, steps: int, cpu_seconds: float = 0.0,
         "spheres": n, "grids": cfg.grids, "spring": cfg.enable_spring, "layout": cfg.layout,
         "time_delta": cfg.time_delta, "steps": steps,
         "tuning": {"scan_margin": tuning[0], "stage_pool": tuning[1],
-                   "row_sort_threads": tuning[2], "dense_scan": tuning[3], "frame_graph": graph},
+                   "row_sort_threads": tuning[2], "dense_scan": tuning[3],
+                   "one_launch": tuning[4], "frame_graph": graph},
         "value": n * steps / el, "unit": "sphere-steps/s", "ms_per_step": el * 1e3 / steps,
         "stage_ms": {"row_sort": float(stage_ms[0]), "collide_integrate": float(stage_ms[1])},
         "collide_roofline_frac": achieved / HBM_PEAK_GBS,
@@ -324,7 +325,8 @@ def main() -> int:
     ap.add_argument("--autotune", action="store_true",
                     help="let the library's autotuner pick the launch choices before timing")
     ap.add_argument("--tuning", default=None,
-                    help="scan_margin,stage_pool,row_sort_threads[,dense_scan] for the measured workload")
+                    help="scan_margin,stage_pool,row_sort_threads[,dense_scan[,one_launch]] "
+                         "for the measured workload")
     ap.add_argument("--frame-graph", action="store_true",
                     help="replay runs of frames as a captured graph instead of launching each kernel")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -354,8 +356,8 @@ def main() -> int:
     if args.frame_graph:
         phys.set_frame_graph(True)
     if args.tuning:
-        m, p, s, d = (args.tuning.split(",") + ["0"])[:4]
-        phys.set_tuning(float(m), int(p), int(s), int(d))
+        m, p, s, d, o = (args.tuning.split(",") + ["0", "0"])[:5]
+        phys.set_tuning(float(m), int(p), int(s), int(d), int(o))
     t0 = time.perf_counter()
     phys.upload_state(pos, vf)  # host -> HBM, synchronised by the library
     upload_ms = (time.perf_counter() - t0) * 1e3
@@ -433,7 +435,7 @@ def main() -> int:
             "parallelism": "single GPU",
             "tuning": None if tuning is None else {
                 "scan_margin": tuning[0], "stage_pool": tuning[1], "row_sort_threads": tuning[2],
-                "dense_scan": tuning[3], "frame_graph": tuning[4]},
+                "dense_scan": tuning[3], "one_launch": tuning[4], "frame_graph": tuning[5]},
         },
         "p50_step_ms": float(np.percentile(lat, 50)),
         "p99_step_ms": float(np.percentile(lat, 99)),

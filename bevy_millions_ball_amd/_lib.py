@@ -51,7 +51,7 @@ class MbBuffers(C.Structure):
 class MbTuning(C.Structure):
     _fields_ = [("scan_margin", C.c_float), ("stage_pool", C.c_uint32),
                 ("row_sort_threads", C.c_uint32), ("frame_graph", C.c_uint32),
-                ("dense_scan", C.c_uint32)]
+                ("dense_scan", C.c_uint32), ("one_launch", C.c_uint32)]
 
 
 class MbServerConfig(C.Structure):
@@ -130,6 +130,7 @@ SIGNATURES = {
     "mb_physics_debug_download": (C.c_int, [_P, _P, _P, _P, _P, _P]),
     "mb_physics_set_profiling": (C.c_int, [_P, C.c_int]),
     "mb_physics_profile_read": (C.c_int, [_P, _P, C.c_uint32, _U32P]),
+    "mb_physics_debug_set_wait_limit": (C.c_int, [_P, C.c_uint32]),
     "mb_last_error": (C.c_char_p, []),
     "mb_radix_sort_workspace_size": (C.c_size_t, [C.c_uint32]),
     "mb_radix_sort": (

@@ -54,17 +54,18 @@ class TuningCache:
             except (OSError, ValueError):
                 self.entries = {}
 
-    def get(self, key: str) -> Optional[Tuple[float, int, int, int]]:
+    def get(self, key: str) -> Optional[Tuple[float, int, int, int, int]]:
         e = self.entries.get(key)
         if not e:
             return None
         return (float(e["scan_margin"]), int(e["stage_pool"]),
-                int(e.get("row_sort_threads", 512)), int(e.get("dense_scan", 0)))
+                int(e.get("row_sort_threads", 512)), int(e.get("dense_scan", 0)),
+                int(e.get("one_launch", 0)))
 
-    def put(self, key: str, tuning: Tuple[float, int, int, int], ms_per_frame: float) -> None:
+    def put(self, key: str, tuning: Tuple[float, int, int, int, int], ms_per_frame: float) -> None:
         self.entries[key] = {"scan_margin": tuning[0], "stage_pool": tuning[1],
                              "row_sort_threads": tuning[2], "dense_scan": tuning[3],
-                             "ms_per_frame": ms_per_frame}
+                             "one_launch": tuning[4], "ms_per_frame": ms_per_frame}
         self.path.parent.mkdir(parents=True, exist_ok=True)
         tmp = self.path.with_suffix(".tmp")
         tmp.write_text(json.dumps(self.entries, indent=1, sort_keys=True))
@@ -72,7 +73,7 @@ class TuningCache:
 
 
 def tune(physics: Physics, cache: Optional[TuningCache] = None,
-         frames_per_trial: int = 5) -> Tuple[Tuple[float, int, int, int], bool]:
+         frames_per_trial: int = 5) -> Tuple[Tuple[float, int, int, int, int], bool]:
     """Applies the cached tuning for the current scene, or times the candidates
     and caches the fastest.  Returns (tuning, came_from_cache)."""
     cache = cache if cache is not None else TuningCache()

@@ -189,6 +189,13 @@ bool finite_half(uint32_t h16) { return ((h16 >> 10) & 0x1Fu) != 0x1Fu; }
 
 }  // namespace
 
+static size_t tile_row_words(uint32_t n) { return 2 * (((size_t)n + 255) / 256); }
+
+// Polls of one bounded wait of the one-launch frame (each poll sleeps about
+// 64 clocks, so this is about a tenth of a second; a row sort takes
+// microseconds).
+constexpr uint32_t kDefaultWaitLimit = 1u << 22;
+
 struct mb_physics {
     mb_settings settings;
     mb::PhysConsts consts;
@@ -209,7 +216,7 @@ struct mb_physics {
     uint32_t *counters = nullptr;
     uint64_t *bucket = nullptr;
     uint32_t bucket_rows = 0, bucket_cap = 0;
-    uint64_t *spill = nullptr;          // N
+    uint64_t *spill = nullptr;          // 2 N: a list per bucket set
     uint64_t *scratch_pairs = nullptr;  // N
     uint32_t *scratch_idx = nullptr;    // N
     // Profiling: three events per frame (before the row sort, before
@@ -251,6 +258,16 @@ struct mb_physics {
     hipStream_t capture_stream = nullptr;
     hipGraphExec_t frame_graph = nullptr;
 
+    // The one-launch frame's row states (kMaxRows words, zeroed at creation
+    // and whenever the launch number wraps), its launch number (the last one
+    // used) and its wait error word (cleared when a state is accepted).
+    uint32_t *row_flags = nullptr;
+    uint32_t *tile_rows = nullptr;   // 2 ceil(N / 256) words, zeroed with row_flags
+    uint32_t one_launch_number = 0;
+    uint32_t *wait_error = nullptr;
+    uint32_t wait_limit = kDefaultWaitLimit;
+    bool one_launch_ran = false;     // a one-launch frame ran since the state was accepted
+
     mb::RowBuckets buckets() const
     {
         mb::RowBuckets rb;
@@ -258,6 +275,7 @@ struct mb_physics {
         rb.spill = spill;
         mb::bucket_counters(counters, mb::kMaxRows, rb);
         rb.cap = bucket_cap;
+        rb.spill_stride = n;  // a spill list per set (the one-launch frame reads one, files the other)
         return rb;
     }
 };
@@ -304,6 +322,9 @@ static void release(mb_physics *p)
     if (p->scratch_idx) (void)hipFree(p->scratch_idx);
     if (p->sort_ws) (void)hipFree(p->sort_ws);
     if (p->check) (void)hipFree(p->check);
+    if (p->wait_error) (void)hipFree(p->wait_error);
+    if (p->row_flags) (void)hipFree(p->row_flags);
+    if (p->tile_rows) (void)hipFree(p->tile_rows);
     if (p->frame_graph) (void)hipGraphExecDestroy(p->frame_graph);
     if (p->capture_stream) (void)hipStreamDestroy(p->capture_stream);
     for (auto &fe : p->prof_events)
@@ -362,6 +383,37 @@ static int sync_last(mb_physics *p)
 {
     MB_HIP(hipStreamSynchronize(p->last_stream));
     return MB_OK;
+}
+
+// Before a one-launch frame: takes its launch number; when the numbers would
+// reach 2^19, zeroes the row states and tile rows (stream ordered, by
+// kernels) and starts again at 1.
+static hipError_t prepare_one_launch(mb_physics *p, hipStream_t st)
+{
+    if (!p->tuning.one_launch) return hipSuccess;
+    p->one_launch_ran = true;
+    if (p->one_launch_number + 1u >= (1u << 19)) {
+        hipError_t e = mb::launch_fill_u32(p->row_flags, 0u, mb::kMaxRows, st);
+        if (e == hipSuccess) e = mb::launch_fill_u32(p->tile_rows, 0u, tile_row_words(p->n), st);
+        if (e != hipSuccess) return e;
+        p->one_launch_number = 0;
+    }
+    ++p->one_launch_number;
+    return hipSuccess;
+}
+
+// After a synchronisation: whether a bounded wait of a one-launch frame ran
+// out since the state was accepted (never expected: every wait is for a row
+// sort that has started).  The state is then not valid.
+static int check_one_launch(const mb_physics *p)
+{
+    if (!p->one_launch_ran) return MB_OK;
+    uint32_t err = 0;
+    MB_HIP(hipMemcpy(&err, p->wait_error, 4, hipMemcpyDeviceToHost));
+    if (err == 0u) return MB_OK;
+    const_cast<mb_physics *>(p)->valid = false;
+    return fail(MB_ERR_STATE, "a bounded wait of a one-launch frame ran out, so the state is "
+                              "not valid: upload a state or load a checkpoint");
 }
 
 static int ensure_sort_ws(mb_physics *p)
@@ -462,10 +514,13 @@ int mb_physics_create(const mb_settings *settings, mb_physics **out)
     ok = ok && hipMalloc(&p->off, (n + 1) * 4u) == hipSuccess;
     ok = ok && hipMalloc(&p->counters, kCounterWords * 4u) == hipSuccess;
     ok = ok && alloc_buckets(p) == MB_OK;
-    ok = ok && hipMalloc(&p->spill, n * 8u) == hipSuccess;
+    ok = ok && hipMalloc(&p->spill, 2 * n * 8u) == hipSuccess;
     ok = ok && hipMalloc(&p->scratch_pairs, n * 8u) == hipSuccess;
     ok = ok && hipMalloc(&p->scratch_idx, n * 4u) == hipSuccess;
     ok = ok && hipMalloc(&p->check, (1 + mb::kStateCheckParts) * sizeof(mb::StateCheck)) == hipSuccess;
+    ok = ok && hipMalloc(&p->wait_error, 4) == hipSuccess;
+    ok = ok && hipMalloc(&p->row_flags, mb::kMaxRows * 4u) == hipSuccess;
+    ok = ok && hipMalloc(&p->tile_rows, tile_row_words(p->n) * 4u) == hipSuccess;
     if (!ok) {
         release(p);
         delete p;
@@ -473,7 +528,10 @@ int mb_physics_create(const mb_settings *settings, mb_physics **out)
     }
     // Buffers start zeroed, as wgpu's do (the INIT frame sorts all-zero keys).
     ok = hipMemset(p->counters, 0, kCounterWords * 4u) == hipSuccess &&
-         hipMemset(p->off, 0, (n + 1) * 4u) == hipSuccess;
+         hipMemset(p->off, 0, (n + 1) * 4u) == hipSuccess &&
+         hipMemset(p->wait_error, 0, 4) == hipSuccess &&
+         hipMemset(p->row_flags, 0, mb::kMaxRows * 4u) == hipSuccess &&
+         hipMemset(p->tile_rows, 0, tile_row_words(p->n) * 4u) == hipSuccess;
     for (int i = 0; i < 2 && ok; ++i) {
         ok = hipMemset(p->pos[i], 0, n * sizeof(float4)) == hipSuccess &&
              hipMemset(p->vfme[i], 0, n * sizeof(uint4)) == hipSuccess &&
@@ -590,6 +648,8 @@ int mb_physics_upload_state_device(mb_physics *p, const void *pos, const void *v
         MB_HIP(hipMemsetAsync(p->vals[i], 0, n * 4u, st));
     }
     MB_HIP(hipMemsetAsync(p->counters, 0, kCounterWords * 4u, st));
+    MB_HIP(hipMemsetAsync(p->wait_error, 0, 4, st));
+    p->one_launch_ran = false;
     StateInfo info;
     int rc = check_state(p, p->consts, p->pos[0], p->vfme[0], nullptr, st, info);
     if (rc != MB_OK) return rc;
@@ -620,7 +680,7 @@ int mb_physics_download_state(const mb_physics *p, float *pos, uint32_t *vfme, v
     if (pos) MB_HIP(hipMemcpyAsync(pos, p->pos[cur], (size_t)p->n * sizeof(float4), hipMemcpyDeviceToHost, st));
     if (vfme) MB_HIP(hipMemcpyAsync(vfme, p->vfme[cur], (size_t)p->n * sizeof(uint4), hipMemcpyDeviceToHost, st));
     MB_HIP(hipStreamSynchronize(st));
-    return MB_OK;
+    return check_one_launch(p);
 }
 
 // ---- checkpoints --------------------------------------------------------
@@ -721,7 +781,8 @@ static int run_frame(mb_physics *p, hipStream_t st)
                                         (uint32_t)par, p->n, st);
         if (e != hipSuccess) return fail_hip(e, "rekey");
     }
-    hipError_t e = launch_frame_path(p, t, st, ev ? ev->data() : nullptr);
+    hipError_t e = prepare_one_launch(p, st);
+    if (e == hipSuccess) e = launch_frame_path(p, t, st, ev ? ev->data() : nullptr);
     if (e != hipSuccess) return fail_hip(e, "frame");
     p->time = t;
     p->filed_for = t + 1;
@@ -745,8 +806,14 @@ static hipError_t launch_frame_path(mb_physics *p, int64_t t, hipStream_t st, hi
     b.vfme_i = p->vfme[par];
     b.pos_o = p->pos[q];
     b.vfme_o = p->vfme[q];
+    b.launch = p->one_launch_number;
+    b.row_flags = p->row_flags;
+    b.tile_rows = p->tile_rows;
+    b.wait_error = p->wait_error;
+    b.wait_limit = p->wait_limit;
     return mb::launch_frame(b, p->consts, t, p->n, st, ev, p->tuning);
 }
+
 
 int mb_physics_step(mb_physics *p, uint32_t frames, void *stream)
 {
@@ -761,7 +828,10 @@ int mb_physics_step(mb_physics *p, uint32_t frames, void *stream)
         const int64_t t = p->time + 1;
         // two frame-path LOOP frames, even then odd, both past frame 1 (whose
         // row sort reads the INIT filing's counters) and already filed
-        const bool pair = p->tuning.frame_graph && !p->profiling && frames - f >= 2 &&
+        // (one-launch frames take their launch number as an argument, so a
+        // replayed capture would reuse one: they are launched directly)
+        const bool pair = p->tuning.frame_graph && !p->tuning.one_launch && !p->profiling &&
+                          frames - f >= 2 &&
                           (t & 1) == 0 && t >= 2 && !p->stage_always && t > p->stage_until &&
                           p->filed_for == t;
         if (!pair) {
@@ -880,6 +950,7 @@ int mb_physics_get_tuning(const mb_physics *p, mb_tuning *t)
     t->row_sort_threads = p->tuning.narrow_sort ? 256u : mb::kSortThreads;
     t->frame_graph = p->tuning.frame_graph ? 1u : 0u;
     t->dense_scan = p->tuning.dense_scan ? 1u : 0u;
+    t->one_launch = p->tuning.one_launch ? 1u : 0u;
     return MB_OK;
 }
 
@@ -894,6 +965,7 @@ int mb_physics_set_tuning(mb_physics *p, const mb_tuning *t)
         return fail(MB_ERR_INVALID_ARGUMENT, "row_sort_threads must be 256 or 512");
     if (t->frame_graph > 1u) return fail(MB_ERR_INVALID_ARGUMENT, "frame_graph must be 0 or 1");
     if (t->dense_scan > 1u) return fail(MB_ERR_INVALID_ARGUMENT, "dense_scan must be 0 or 1");
+    if (t->one_launch > 1u) return fail(MB_ERR_INVALID_ARGUMENT, "one_launch must be 0 or 1");
     drop_graph(p);
     p->scan_margin = t->scan_margin;
     p->consts.scan_margin = t->scan_margin;
@@ -901,6 +973,7 @@ int mb_physics_set_tuning(mb_physics *p, const mb_tuning *t)
     p->tuning.narrow_sort = t->row_sort_threads == 256u;
     p->tuning.frame_graph = t->frame_graph != 0u;
     p->tuning.dense_scan = t->dense_scan != 0u;
+    p->tuning.one_launch = t->one_launch != 0u;
     return MB_OK;
 }
 
@@ -933,6 +1006,7 @@ int mb_physics_autotune(mb_physics *p, uint32_t frames, mb_tuning *best, double 
         float ms = 0.0f;
         if (rc == MB_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
         if (rc == MB_OK && e != hipSuccess) rc = fail_hip(e, "autotune timing");
+        if (rc == MB_OK) rc = check_one_launch(p);
         per = (double)ms / fr;
         return rc;
     };
@@ -945,22 +1019,27 @@ int mb_physics_autotune(mb_physics *p, uint32_t frames, mb_tuning *best, double 
     };
     // every candidate for `frames` frames, then the four fastest again over
     // four times as many (short trials cannot tell apart choices a few
-    // per cent apart), then the winner against its other frame_graph choice
+    // per cent apart), then the winner against its other frame_graph choice.
+    // The one-launch frame always sorts rows with 256-thread blocks, so its
+    // candidates do not vary the row sort width.
     const float margins[] = {1.0f, 0.5f, 0.25f, 0.0625f};
     const uint32_t pools[] = {mb::kStagePool, mb::kStagePoolBig};
     const uint32_t sorts[] = {mb::kSortThreads, 256u};
     const uint32_t dense[] = {0u, 1u};
+    const uint32_t launches[] = {0u, 1u};
     std::vector<std::pair<double, mb_tuning>> coarse;
-    for (uint32_t ds : dense)
-        for (uint32_t sort : sorts)
-            for (uint32_t pool : pools)
-                for (float margin : margins) {
-                    const mb_tuning t{margin, pool, sort, keep.frame_graph, ds};
-                    double per = 0.0;
-                    const int rc = trial(t, frames, per);
-                    if (rc != MB_OK) return give_up(rc);
-                    coarse.emplace_back(per, t);
-                }
+    for (uint32_t one : launches)
+        for (uint32_t ds : dense)
+            for (uint32_t sort : sorts)
+                for (uint32_t pool : pools)
+                    for (float margin : margins) {
+                        if (one && sort != 256u) continue;
+                        const mb_tuning t{margin, pool, sort, keep.frame_graph, ds, one};
+                        double per = 0.0;
+                        const int rc = trial(t, frames, per);
+                        if (rc != MB_OK) return give_up(rc);
+                        coarse.emplace_back(per, t);
+                    }
     std::stable_sort(coarse.begin(), coarse.end(),
                      [](const std::pair<double, mb_tuning> &x,
                         const std::pair<double, mb_tuning> &y) { return x.first < y.first; });
@@ -1064,6 +1143,8 @@ int mb_physics_debug_download(const mb_physics *p, uint32_t *perm, uint32_t *sta
     if (perm) MB_HIP(hipMemcpyAsync(perm, p->vals[q], n * 4u, hipMemcpyDeviceToHost, st));
     if (next_keys) MB_HIP(hipMemcpyAsync(next_keys, p->keys[q], n * 4u, hipMemcpyDeviceToHost, st));
     MB_HIP(hipStreamSynchronize(st));
+    const int rc = check_one_launch(p);
+    if (rc != MB_OK) return rc;
     // start/close as spatial_hashing.wgsl leaves them: start is INVALID for
     // empty cells (and for the entries past the last cell); close is reported
     // only where start is valid (0 elsewhere).
@@ -1076,6 +1157,14 @@ int mb_physics_debug_download(const mb_physics *p, uint32_t *perm, uint32_t *sta
         if (start) start[c] = in ? off[c] : MB_INVALID_INDEX;
         if (close) close[c] = in ? off[c + 1] - 1u : 0u;
     }
+    return MB_OK;
+}
+
+int mb_physics_debug_set_wait_limit(mb_physics *p, uint32_t polls)
+{
+    if (!p) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
+    drop_graph(p);
+    p->wait_limit = polls ? polls : kDefaultWaitLimit;
     return MB_OK;
 }
 
@@ -1092,6 +1181,7 @@ int mb_physics_profile_read(mb_physics *p, double *stage_ms, uint32_t capacity,
     if (!p || !frames) return fail(MB_ERR_INVALID_ARGUMENT, "null argument");
     DeviceGuard g(p->device);
     int rc = drain_profile(p);
+    if (rc == MB_OK) rc = check_one_launch(p);
     if (rc != MB_OK) return rc;
     const size_t have = p->prof_ms.size() / 2;
     const size_t take = have < capacity ? have : capacity;

@@ -146,7 +146,9 @@ constexpr uint32_t kStagePoolBig = 1408; // the same for dense scenes (fewer blo
 // frame parity: the row sort of frame t reads set t & 1 (and the prefix of
 // the fills of the rows before its own) while it clears set !(t & 1), which
 // the collide of frame t then fills.  Pairs beyond a bucket's capacity spill
-// into one list (row << 42 | column << 30 | slot).
+// into a list of their set (row << 42 | column << 30 | slot): set s appends
+// at spill + s * spill_stride, so that the row sort of a frame can read the
+// list of one set while the filing of the same frame writes the other's.
 // The counters are packed (kFillStride = 1 word apart): the collide
 // kernel reserves with one returning atomic per row per block, so neighbouring
 // rows sharing a line costs it nothing measurable, while the row sort's
@@ -163,11 +165,12 @@ constexpr uint32_t kSpread = 32;
 
 struct RowBuckets {
     uint64_t *bucket;         // G x cap pairs (column << 32 | slot)
-    uint64_t *spill;          // n entries
+    uint64_t *spill;          // the spill lists of both sets, n entries each
     uint32_t *fill;           // 2 x G counters, set-major, kFillStride words apart
     uint32_t *spill_count;    // 2 counters, kFillStride words apart
     uint32_t *spread;         // G counters, kSpread words apart: filings in input order
     uint32_t cap;             // pairs per bucket
+    uint32_t spill_stride;    // entries from set 0's spill list to set 1's (0: one shared list)
 };
 
 // Words of the counters of G rows: the spread set, both packed sets, both
@@ -206,6 +209,16 @@ struct FrameBuffers {
     const uint4 *vfme_i;
     float4 *pos_o;
     uint4 *vfme_o;
+    // The one-launch frame (FrameTuning::one_launch): launch number T >= 1,
+    // distinct for every one-launch frame of these buffers since row_flags was
+    // zeroed (row r's word reads 2T + 1 while it is sorted and 2T + 2 once it
+    // is); a wait polls at most wait_limit times, and one that gives up sets
+    // *wait_error (the results are then not valid).
+    uint32_t launch;           // below 2^19
+    uint32_t *row_flags;       // kMaxRows words
+    uint32_t *tile_rows;       // 2 ceil(N / 256) words: rows of each tile's first and last slot
+    uint32_t *wait_error;
+    uint32_t wait_limit;
 };
 
 // Launch choices of a frame that do not change results.
@@ -214,13 +227,16 @@ struct FrameTuning {
     bool narrow_sort = false;  // row sort blocks of 256 threads keeping kRowCapNarrow spheres in LDS
     bool frame_graph = false;  // runs of LOOP frames replay a captured two-frame hipGraph
     bool dense_scan = false;   // collide variant whose dense blocks take the split candidate scan
+    bool one_launch = false;   // LOOP frames as one launch: row sorts, then tiles that wait per row
 };
 
 // Enqueues frame `time`: INIT at time 0 (offsets, copy, keys, buckets for
 // frame 1 counted in the spread set), else LOOP (row sort of bucket set
 // time & 1, counted in the spread set at time 1, collide + integrate filing
-// set !(time & 1)).  `events` (3 entries or null) are recorded before
-// the row sort, before collide + integrate and after it.
+// set !(time & 1)), as two launches or, with tuning.one_launch, as one.
+// `events` (3 entries or null) are recorded before the row sort, before
+// collide + integrate and after it (with one launch, the first two
+// together).
 hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time, uint32_t n,
                         hipStream_t stream, hipEvent_t *events, FrameTuning tuning = {});
 

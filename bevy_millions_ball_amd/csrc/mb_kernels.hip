@@ -1060,12 +1060,26 @@ __device__ __forceinline__ uint32_t block_scan_lds(uint32_t *vals, uint32_t coun
 // theirs at once; where most neighbouring lanes differ (input in upload
 // order), every lane takes its own.  The row sort does not depend on the
 // order inside a bucket.  A pair beyond the bucket's capacity goes to the
-// spill list.  Every active lane files one pair.
-template <uint32_t STRIDE>
+// spill list `spill`.  Every active lane files one pair.
+//
+// `may_file(row)` says whether the lane may file under `row` now; the
+// single-scene and batched kernels always may (NoWait).  In the one-launch
+// frame it waits until that row's sort has read its bucket and cleared the
+// counter this filing increments, and it is false only when that bounded
+// wait ran out (the lane then files nothing).  Spill count and list are
+// cleared by row 0's sort, so a spill asks for row 0.
+struct NoWait {
+    __device__ __forceinline__ bool operator()(uint32_t) const { return true; }
+};
+
+template <uint32_t STRIDE, typename MayFile = NoWait>
 __device__ __forceinline__ void bucket_file(const RowBuckets &rb, uint32_t *__restrict__ fill,
-                                            uint32_t *__restrict__ spill_count, uint32_t row,
-                                            uint32_t col, uint32_t slot)
+                                            uint32_t *__restrict__ spill_count,
+                                            uint64_t *__restrict__ spill, uint32_t row,
+                                            uint32_t col, uint32_t slot,
+                                            const MayFile &may_file = MayFile())
 {
+    if (!may_file(row)) return;
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t lt = lanemask_lt(lane);
     uint64_t todo = __ballot(1);
@@ -1097,9 +1111,9 @@ __device__ __forceinline__ void bucket_file(const RowBuckets &rb, uint32_t *__re
     }
     if (at < rb.cap) {
         rb.bucket[(size_t)row * rb.cap + at] = ((uint64_t)col << 32) | slot;
-    } else {
+    } else if (may_file(0u)) {
         const uint32_t o = atomicAdd(spill_count, 1u);
-        rb.spill[o] = ((uint64_t)row << 42) | ((uint64_t)col << 30) | slot;
+        spill[o] = ((uint64_t)row << 42) | ((uint64_t)col << 30) | slot;
     }
 }
 
@@ -1121,10 +1135,13 @@ struct BlockFiling {
 
 // Called by the active lanes of every wave that holds a valid slot (slots
 // s0 .. s0 + last); `row0` is the row of the block's first slot.
+template <typename MayFile = NoWait>
 __device__ __forceinline__ void block_file(BlockFiling &f, const RowBuckets &rb,
                                            uint32_t *__restrict__ fill,
-                                           uint32_t *__restrict__ spill_count, uint32_t row0,
-                                           uint32_t row, uint32_t col, uint32_t s0, uint32_t last)
+                                           uint32_t *__restrict__ spill_count,
+                                           uint64_t *__restrict__ spill, uint32_t row0,
+                                           uint32_t row, uint32_t col, uint32_t s0, uint32_t last,
+                                           const MayFile &may_file = MayFile())
 {
     const uint32_t tid = threadIdx.x;
     const uint32_t w = row + kFileLo - row0;  // u32: rows outside the window wrap high
@@ -1133,7 +1150,7 @@ __device__ __forceinline__ void block_file(BlockFiling &f, const RowBuckets &rb,
         const uint32_t rank = atomicAdd(&f.count[w], 1u);
         entry = (w << 24) | (rank << 12) | col;
     } else {
-        bucket_file<kFillStride>(rb, fill, spill_count, row, col, s0 + tid);
+        bucket_file<kFillStride>(rb, fill, spill_count, spill, row, col, s0 + tid, may_file);
     }
     f.entry[tid] = entry;
     __threadfence_block();
@@ -1149,21 +1166,24 @@ __device__ __forceinline__ void block_file(BlockFiling &f, const RowBuckets &rb,
     const uint32_t nact = (uint32_t)__popcll(act);
     for (uint32_t i = li; i < kFileWin; i += nact) {
         const uint32_t cnt = f.count[i];
-        if (cnt) f.base[i] = atomicAdd(&fill[(row0 + i - kFileLo) * kFillStride], cnt);
+        const uint32_t r = row0 + i - kFileLo;
+        // a row whose wait ran out gets none of the block's pairs
+        if (cnt) f.base[i] = may_file(r) ? atomicAdd(&fill[r * kFillStride], cnt) : kInvalid;
     }
     __threadfence_block();
     for (uint32_t i = li; i <= last; i += nact) {
         const uint32_t e = f.entry[i];
         if (e == kInvalid) continue;
         const uint32_t wr = e >> 24;
+        if (f.base[wr] == kInvalid) continue;
         const uint32_t at = f.base[wr] + ((e >> 12) & 0xFFFu);
         const uint32_t r = row0 + wr - kFileLo;
         const uint32_t cl = e & 0xFFFu;
         if (at < rb.cap) {
             rb.bucket[(size_t)r * rb.cap + at] = ((uint64_t)cl << 32) | (s0 + i);
-        } else {
+        } else if (may_file(0u)) {
             const uint32_t o = atomicAdd(spill_count, 1u);
-            rb.spill[o] = ((uint64_t)r << 42) | ((uint64_t)cl << 30) | (s0 + i);
+            spill[o] = ((uint64_t)r << 42) | ((uint64_t)cl << 30) | (s0 + i);
         }
     }
 }
@@ -1182,32 +1202,84 @@ __host__ __device__ constexpr uint32_t row_sort_cells(uint32_t grids, uint32_t n
 // then each index is ranked among the indices of its cell, which makes the
 // result exactly the stable argsort of the keys.  Writes perm, the sorted
 // keys and the per-cell offsets of the row, and clears bucket set
-// !(time & 1) for the collide kernel that follows.  Dynamic LDS: `grids` cell counters (padded
-// to a multiple of 1024 words for the vector scan) followed by kRowCap
-// indices.
-template <uint32_t NT, uint32_t CAP = kRowCap>
-__device__ __forceinline__ void row_sort_body(
+// !(time & 1) for the collide kernel that follows.  Besides the offsets of
+// its own cells it writes off[(r + 1) G], the end of the row, which is the
+// same value row r + 1 writes there as its start: a reader of a row's end
+// then needs only that row.
+// LDS (RowSortLds): `grids` cell counters (padded to a multiple of 4 NT words
+// for the vector scan), CAP indices, NT / 64 scan words and a spill count.
+//
+// SC: the row's outputs (perm, sorted keys, offsets, the cleared counter and
+// spill count of the set the frame files) are agent-scope stores, which are
+// coherent across the XCDs' L2 caches when they complete, for readers in the
+// same launch (the one-launch frame); its readers use the matching loads
+// (ld_sc).  Plain stores otherwise.
+template <bool SC>
+__device__ __forceinline__ void st_sc(uint32_t *p, uint32_t v)
+{
+    if (SC)
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        *p = v;
+}
+
+template <bool SC>
+__device__ __forceinline__ uint32_t ld_sc(const uint32_t *p)
+{
+    if (SC) return __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+    return *p;
+}
+
+struct RowSortLds {
+    uint32_t *cell;
+    uint32_t *idx;
+    uint32_t *scratch;
+    uint32_t *spilled;
+};
+
+template <uint32_t NT, uint32_t CAP>
+__host__ __device__ constexpr uint32_t row_sort_lds_words(uint32_t grids)
+{
+    return row_sort_cells(grids, NT) + CAP + NT / 64u + 1u;
+}
+
+template <uint32_t NT, uint32_t CAP>
+__device__ __forceinline__ RowSortLds row_sort_lds_at(uint32_t *base, uint32_t grids)
+{
+    RowSortLds l;
+    l.cell = base;
+    l.idx = base + row_sort_cells(grids, NT);
+    l.scratch = l.idx + CAP;
+    l.spilled = l.scratch + NT / 64u;
+    return l;
+}
+
+// Returns the row's first slot and its sphere count.
+template <uint32_t NT, uint32_t CAP = kRowCap, bool SC = false>
+__device__ __forceinline__ uint2 row_sort_body(
     const RowBuckets &rb, uint32_t set, bool spread_in, uint64_t *__restrict__ scratch_pairs,
     uint32_t *__restrict__ scratch_idx, uint32_t grids, uint32_t n,
     uint32_t *__restrict__ perm, uint32_t *__restrict__ off, uint32_t *__restrict__ sorted_keys,
-    uint32_t r)
+    uint32_t r, const RowSortLds &lds)
 {
-    extern __shared__ __align__(16) uint32_t s_dyn[];
-    uint32_t *s_cell = s_dyn;                              // grids words (padded)
-    uint32_t *s_idx = s_dyn + row_sort_cells(grids, NT);  // kRowCap words
-    __shared__ uint32_t s_scratch[NT / 64u];
-    __shared__ uint32_t s_spilled;
+    uint32_t *s_cell = lds.cell;      // grids words (padded)
+    uint32_t *s_idx = lds.idx;        // CAP words
+    uint32_t *s_scratch = lds.scratch;
+    uint32_t &s_spilled = *lds.spilled;
     const uint32_t tid = threadIdx.x;
     const uint32_t *fill_in = spread_in ? rb.spread : rb.fill + set * grids * kFillStride;
     const uint32_t stride = spread_in ? kSpread : kFillStride;
+    // the counters of the rows below, four loads in flight at a time
     uint32_t part = 0;
+#pragma unroll 4
     for (uint32_t q = tid; q < r; q += NT) part += fill_in[q * stride];
     uint32_t base;
     (void)block_exclusive_scan<NT>(part, s_scratch, base);  // base = spheres in rows < r
     const uint32_t m = fill_in[r * stride];
     if (tid == 0) {
-        rb.fill[((set ^ 1u) * grids + r) * kFillStride] = 0u;
-        if (r == 0) rb.spill_count[(set ^ 1u) * kFillStride] = 0u;
+        st_sc<SC>(&rb.fill[((set ^ 1u) * grids + r) * kFillStride], 0u);
+        if (r == 0) st_sc<SC>(&rb.spill_count[(set ^ 1u) * kFillStride], 0u);
         s_spilled = 0u;
     }
     for (uint32_t c = tid; c < grids; c += NT) s_cell[c] = 0u;
@@ -1218,8 +1290,9 @@ __device__ __forceinline__ void row_sort_body(
         for (uint32_t e = tid; e < rb.cap; e += NT) scratch_pairs[base + e] = src[e];
         __syncthreads();
         const uint32_t spilled = rb.spill_count[set * kFillStride];
+        const uint64_t *spill = rb.spill + (size_t)set * rb.spill_stride;
         for (uint32_t i = tid; i < spilled; i += NT) {
-            const uint64_t v = rb.spill[i];
+            const uint64_t v = spill[i];
             if ((uint32_t)(v >> 42) == r) {
                 const uint32_t at = rb.cap + atomicAdd(&s_spilled, 1u);
                 scratch_pairs[base + at] = (((v >> 30) & 0xFFFull) << 32) | (v & 0x3FFFFFFFull);
@@ -1245,7 +1318,7 @@ __device__ __forceinline__ void row_sort_body(
         }
         __syncthreads();
         (void)block_scan_lds<NT>(s_cell, grids, s_scratch);
-        for (uint32_t c = tid; c < grids; c += NT) off[r * grids + c] = base + s_cell[c];
+        for (uint32_t c = tid; c < grids; c += NT) st_sc<SC>(&off[r * grids + c], base + s_cell[c]);
         __syncthreads();
 #pragma unroll
         for (uint32_t j = 0; j < kPer; ++j) {
@@ -1275,8 +1348,8 @@ __device__ __forceinline__ void row_sort_body(
                     }
                 }
                 for (; q < ce; ++q) rank += s_idx[q] < idx;
-                perm[base + cs + rank] = idx;
-                sorted_keys[base + cs + rank] = r * grids + c;
+                st_sc<SC>(&perm[base + cs + rank], idx);
+                st_sc<SC>(&sorted_keys[base + cs + rank], r * grids + c);
             }
         }
     } else {
@@ -1286,7 +1359,7 @@ __device__ __forceinline__ void row_sort_body(
             atomicAdd(&s_cell[(uint32_t)(src[e] >> 32)], 1u);
         __syncthreads();
         (void)block_scan_lds<NT>(s_cell, grids, s_scratch);
-        for (uint32_t c = tid; c < grids; c += NT) off[r * grids + c] = base + s_cell[c];
+        for (uint32_t c = tid; c < grids; c += NT) st_sc<SC>(&off[r * grids + c], base + s_cell[c]);
         __syncthreads();
         for (uint32_t e = tid; e < m; e += NT) {
             const uint64_t v = src[e];
@@ -1302,11 +1375,12 @@ __device__ __forceinline__ void row_sort_body(
             const uint32_t ce = s_cell[c];
             uint32_t rank = 0;
             for (uint32_t q = cs; q < ce; ++q) rank += scratch_idx[base + q] < idx;
-            perm[base + cs + rank] = idx;
-            sorted_keys[base + cs + rank] = r * grids + c;
+            st_sc<SC>(&perm[base + cs + rank], idx);
+            st_sc<SC>(&sorted_keys[base + cs + rank], r * grids + c);
         }
     }
-    if (tid == 0 && r == grids - 1u) off[grids * grids] = n;
+    if (tid == 0) st_sc<SC>(&off[(r + 1u) * grids], r == grids - 1u ? n : base + m);
+    return make_uint2(base, m);
 }
 
 template <uint32_t NT, uint32_t CAP>
@@ -1315,14 +1389,16 @@ __global__ __launch_bounds__(NT) void row_sort_kernel(
     uint32_t *__restrict__ scratch_idx, uint32_t grids, uint32_t n,
     uint32_t *__restrict__ perm, uint32_t *__restrict__ off, uint32_t *__restrict__ sorted_keys)
 {
+    extern __shared__ __align__(16) uint32_t s_dyn[];
     row_sort_body<NT, CAP>(rb, set, spread_in != 0u, scratch_pairs, scratch_idx, grids, n, perm,
-                           off, sorted_keys, xcd_block(blockIdx.x, gridDim.x));
+                           off, sorted_keys, xcd_block(blockIdx.x, gridDim.x),
+                           row_sort_lds_at<NT, CAP>(s_dyn, grids));
 }
 
 template <uint32_t NT = kSortThreads, uint32_t CAP = kRowCap>
 static size_t row_sort_lds(uint32_t grids)
 {
-    return ((size_t)row_sort_cells(grids, NT) + CAP) * 4u;
+    return (size_t)row_sort_lds_words<NT, CAP>(grids) * 4u;
 }
 
 // solve_sphere_constraint behind a cheap conservative filter:
@@ -1401,7 +1477,7 @@ __device__ __forceinline__ Ranges make_ranges(uint32_t lo0, uint32_t hi0, uint32
 // A chunk takes up to 32 candidates across the three ranges (about one
 // sphere per cell, ranges hold a few candidates each); crowded waves take
 // walk_dense instead.
-template <bool SPRING, bool FROM_LDS>
+template <bool SPRING, bool FROM_LDS, bool SC = false>
 __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, uint32_t s,
                                                 const Ranges r,
                                                 const float4 *__restrict__ s_pos,
@@ -1446,7 +1522,7 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
             }
         } else {
             for (uint32_t t = j; t < end; ++t) {
-                const float4 bp = pos_i[perm[slot_of(t)]];
+                const float4 bp = pos_i[ld_sc<SC>(&perm[slot_of(t)])];
                 const float reach = __builtin_fmaf(bp.w, 1.0001f, own_reach_w);
                 const float dx = rx - bp.x, dy = ry - bp.y, dz = rz - bp.z;
                 const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
@@ -1464,7 +1540,7 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
                 const uint32_t l = lds_of(t);
                 moved = collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
             } else {
-                const uint32_t bo = perm[slot_of(t)];
+                const uint32_t bo = ld_sc<SC>(&perm[slot_of(t)]);
                 moved = collide_candidate<SPRING>(c, a, pos_i[bo],
                                                   [&]() { return vel_mass_of(vfme_i[bo]); });
             }
@@ -1613,15 +1689,51 @@ __device__ __forceinline__ void walk_dense(const PhysConsts &c, Agent &a, uint32
 // walk_dense, and the block files its pool in the height-slab bitmaps (a
 // kernel variant of its own, so that scenes near one sphere per cell run the
 // code without it).
-template <bool SPRING, uint32_t POOL, bool PRIO = false, bool DENSE = false>
+// LDS of a collide block: the staging pool, the slab bitmaps (DENSE), the
+// filing tables and the waves' largest radii.
+struct CollideLds {
+    float4 *pos;
+    uint2 *vm;
+    uint32_t *slab;
+    BlockFiling *file;
+    float *wave_rmax;
+};
+
+template <uint32_t POOL, bool DENSE>
+__host__ __device__ constexpr uint32_t collide_lds_bytes()
+{
+    return POOL * 24u + (DENSE ? kSlabs * slab_words(POOL) * 4u : 0u) +
+           (uint32_t)sizeof(BlockFiling) + 16u;
+}
+
+template <uint32_t POOL, bool DENSE>
+__device__ __forceinline__ CollideLds collide_lds_at(void *base)
+{
+    CollideLds l;
+    char *p = static_cast<char *>(base);
+    l.pos = reinterpret_cast<float4 *>(p);
+    l.vm = reinterpret_cast<uint2 *>(p + POOL * 16u);
+    l.slab = reinterpret_cast<uint32_t *>(p + POOL * 24u);
+    l.file = reinterpret_cast<BlockFiling *>(p + POOL * 24u +
+                                             (DENSE ? kSlabs * slab_words(POOL) * 4u : 0u));
+    l.wave_rmax = reinterpret_cast<float *>(l.file + 1);
+    return l;
+}
+
+template <bool SPRING, uint32_t POOL, bool PRIO = false, bool DENSE = false,
+          typename MayFile = NoWait, bool SC = false>
 __device__ __forceinline__ void physics_step_body(
     const PhysConsts &c, const uint32_t *__restrict__ perm, const uint32_t *__restrict__ off,
     uint32_t *__restrict__ keys, const float4 *__restrict__ pos_i,
     const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
-    const RowBuckets &rb, uint32_t set_out, uint32_t n, uint32_t blk,
-    float4 *__restrict__ s_pos, uint2 *__restrict__ s_vm, uint32_t *__restrict__ s_slab)
+    const RowBuckets &rb, uint32_t set_out, uint32_t n, uint32_t blk, const CollideLds &lds,
+    const MayFile &may_file = MayFile())
 {
-    __shared__ BlockFiling s_file;
+    float4 *__restrict__ s_pos = lds.pos;
+    uint2 *__restrict__ s_vm = lds.vm;
+    uint32_t *__restrict__ s_slab = lds.slab;
+    BlockFiling &s_file = *lds.file;
+    float *s_wave_rmax = lds.wave_rmax;
     if (PRIO) __builtin_amdgcn_s_setprio(2);
     const uint32_t tid = threadIdx.x;
     constexpr uint32_t sw = slab_words(POOL);
@@ -1639,46 +1751,51 @@ __device__ __forceinline__ void physics_step_body(
     // `keys` holds the sorted keys on entry (the cell of each slot's sphere,
     // computed from the same position by the previous frame) and receives
     // the next keys; every slot is read and then written by its own thread.
-    const uint32_t k = keys[s];
-    const uint32_t kfirst = keys[s0];
-    const uint32_t klast = keys[s0 + last];
-    const uint32_t o = perm[s];
+    const uint32_t k = ld_sc<SC>(&keys[s]);
+    const uint32_t kfirst = ld_sc<SC>(&keys[s0]);
+    const uint32_t klast = ld_sc<SC>(&keys[s0 + last]);
+    const uint32_t o = ld_sc<SC>(&perm[s]);
     Agent a = agent_from(pos_i[o], vfme_i[o]);
     const uint32_t cz = fast_div(k, c.grids_div);
     const uint32_t cx = k - cz * G;
 
     // windows: every slot of cells [kfirst + dk - 1, klast + dk + 1] for
-    // dk = 0, -G, +G, a superset of each thread's range in that row
+    // dk = 0, -G, +G, without the cell left of column 0 and the cell right of
+    // column G - 1 (no range reaches them): a superset of each thread's range
+    // in that row whose offsets all belong to the rows dk / G away from the
+    // block's rows (the end of a row is written by that row's sort too)
+    const uint32_t xfirst = kfirst - fast_div(kfirst, c.grids_div) * G;
+    const uint32_t xlast = klast - fast_div(klast, c.grids_div) * G;
+    const int64_t wl = xfirst > 0u ? 1 : 0, wh = xlast < G - 1u ? 2 : 1;
     uint32_t wlo[3], wcnt[3];
 #pragma unroll
     for (uint32_t w = 0; w < 3; ++w) {
         const int64_t dk = w == 0 ? 0 : (w == 1 ? -(int64_t)G : (int64_t)G);
-        int64_t lc = (int64_t)kfirst + dk - 1;
-        int64_t hc = (int64_t)klast + dk + 2;
+        int64_t lc = (int64_t)kfirst + dk - wl;
+        int64_t hc = (int64_t)klast + dk + wh;
         lc = lc < 0 ? 0 : (lc > (int64_t)C ? (int64_t)C : lc);
         hc = hc < 0 ? 0 : (hc > (int64_t)C ? (int64_t)C : hc);
-        const uint32_t lo = off[lc];
+        const uint32_t lo = ld_sc<SC>(&off[lc]);
         wlo[w] = lo;
-        wcnt[w] = max(off[hc], lo) - lo;
+        wcnt[w] = max(ld_sc<SC>(&off[hc]), lo) - lo;
     }
     // own ranges of physics.wgsl:123-160 as half-open slot intervals
     const uint32_t lft = cx > 0u ? 1u : 0u;
     const uint32_t rgt = cx < G - 1u ? 2u : 1u;
-    const uint32_t lo0 = off[k - lft], hi0 = off[k + rgt];
+    const uint32_t lo0 = ld_sc<SC>(&off[k - lft]), hi0 = ld_sc<SC>(&off[k + rgt]);
     uint32_t lo1 = 0, hi1 = 0, lo2 = 0, hi2 = 0;
     if (cz > 0u) {
-        lo1 = off[k - G - lft];
-        hi1 = off[k - G + rgt];
+        lo1 = ld_sc<SC>(&off[k - G - lft]);
+        hi1 = ld_sc<SC>(&off[k - G + rgt]);
     }
     if (cz < G - 1u) {
-        lo2 = off[k + G - lft];
-        hi2 = off[k + G + rgt];
+        lo2 = ld_sc<SC>(&off[k + G - lft]);
+        hi2 = ld_sc<SC>(&off[k + G + rgt]);
     }
     // the three windows share one pool, back to back
     const uint32_t wbase[3] = {0u, wcnt[0], wcnt[0] + wcnt[1]};
     const uint32_t wtot = wcnt[0] + wcnt[1] + wcnt[2];
     const bool staged = wtot <= POOL;
-    __shared__ float s_wave_rmax[4];
     // slab height 2H / kSlabs
     const float inv_h = (float)kSlabs / (2.0f * c.half_map_height);
     if (DENSE) __syncthreads();  // s_slab cleared before the staging files into it
@@ -1689,8 +1806,8 @@ __device__ __forceinline__ void physics_step_body(
         const uint32_t d2 = (wlo[2] - wbase[2]) - (wlo[1] - wbase[1]);
         float rmax = -INFINITY;
         for (uint32_t j = tid; j < wtot; j += 256u) {
-            const uint32_t bo =
-                perm[j + wlo[0] + (j >= wbase[1] ? d1 : 0u) + (j >= wbase[2] ? d2 : 0u)];
+            const uint32_t bo = ld_sc<SC>(
+                &perm[j + wlo[0] + (j >= wbase[1] ? d1 : 0u) + (j >= wbase[2] ? d2 : 0u)]);
             const float4 bp = pos_i[bo];
             s_pos[j] = bp;
             if (DENSE) {
@@ -1728,14 +1845,14 @@ __device__ __forceinline__ void physics_step_body(
     } else if (staged) {
         walk_candidates<SPRING, true>(c, a, s, rg, s_pos, s_vm, perm, pos_i, vfme_i);
     } else {
-        walk_candidates<SPRING, false>(c, a, s, rg, s_pos, s_vm, perm, pos_i, vfme_i);
+        walk_candidates<SPRING, false, SC>(c, a, s, rg, s_pos, s_vm, perm, pos_i, vfme_i);
     }
     jitter(a, s, ox, oy, oz);
     const uint32_t next = finish_agent<SPRING, true>(c, a, s, pos_o, vfme_o, keys);
     const uint32_t nz = fast_div(next, c.grids_div);
     block_file(s_file, rb, rb.fill + set_out * G * kFillStride,
-               rb.spill_count + set_out * kFillStride, fast_div(kfirst, c.grids_div), nz,
-               next - nz * G, s0, last);
+               rb.spill_count + set_out * kFillStride, rb.spill + set_out * rb.spill_stride,
+               fast_div(kfirst, c.grids_div), nz, next - nz * G, s0, last, may_file);
 }
 
 template <bool SPRING, uint32_t POOL, bool PRIO, bool DENSE>
@@ -1745,12 +1862,10 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
     RowBuckets rb, uint32_t set_out, uint32_t n)
 {
-    __shared__ float4 s_pos[POOL];  // the three windows back to back
-    __shared__ uint2 s_vm[POOL];    // their velocity and mass halves
-    __shared__ uint32_t s_slab[DENSE ? kSlabs * slab_words(POOL) : 1u];  // height-slab bitmaps
+    __shared__ __align__(16) char s_lds[collide_lds_bytes<POOL, DENSE>()];
     physics_step_body<SPRING, POOL, PRIO, DENSE>(c, perm, off, keys, pos_i, vfme_i, pos_o, vfme_o,
                                                  rb, set_out, n, xcd_block(blockIdx.x, gridDim.x),
-                                                 s_pos, s_vm, s_slab);
+                                                 collide_lds_at<POOL, DENSE>(s_lds));
 }
 
 // INIT frame of the frame path: the sort of the all-zero keys is the
@@ -1771,7 +1886,9 @@ __device__ __forceinline__ void physics_init_body(
     vfme_o[s] = make_uint4(pack_half2(a.vx, a.vy), pack_half2(a.vz, a.fx),
                            pack_half2(a.fy, a.fz), pack_half2(a.m, 0.0f));
     key_out[s] = cz * c.grids + cx;
-    bucket_file<kSpread>(rb, rb.spread, rb.spill_count + kFillStride, cz, cx, s);  // read by frame 1
+    // read by frame 1, whose set is 1
+    bucket_file<kSpread>(rb, rb.spread, rb.spill_count + kFillStride, rb.spill + rb.spill_stride,
+                         cz, cx, s);
 }
 
 __global__ __launch_bounds__(256) void physics_init_step_kernel(
@@ -1825,7 +1942,8 @@ __global__ __launch_bounds__(256) void rekey_kernel(PhysConsts c, const float4 *
     uint32_t cx, cz;
     cell_in_grid(c, p.x, p.z, cx, cz);
     key_out[s] = cz * c.grids + cx;
-    bucket_file<kSpread>(rb, rb.spread, rb.spill_count + set * kFillStride, cz, cx, s);
+    bucket_file<kSpread>(rb, rb.spread, rb.spill_count + set * kFillStride,
+                         rb.spill + set * rb.spill_stride, cz, cx, s);
 }
 
 template <uint32_t POOL, bool PRIO, bool DENSE>
@@ -1859,6 +1977,252 @@ static void launch_collide(const FrameBuffers &b, const PhysConsts &c, uint32_t 
     }
 }
 
+// ---- the one-launch frame ----------------------------------------------------
+// Row sorts and collide tiles of one LOOP frame in a single launch of
+// G + ceil(N / 256) blocks: block b < G sorts a row, block G + j runs collide
+// tile j.  A tile starts as soon as the rows around its slots are sorted, so
+// the tiles of the first rows overlap the sorts of the later ones, and the
+// launch's first round of gathers is spread over the row sorts' finishing
+// times.  Rows are numbered XCD-aware like the two-launch kernels' blocks, so
+// a row is sorted on the XCD whose tiles read it.
+//
+// Each row has a state word per launch number T (T >= 1, from the host):
+// below 2T + 1 unclaimed, 2T + 1 claimed (being sorted), 2T + 2 sorted.  A
+// row's own block claims it with an atomic max; a tile that needs a row that
+// nobody has claimed yet claims and sorts it itself.  A block waits only for
+// rows another block has claimed, and a block that holds a claim sorts the
+// row before it waits for anything: no wait depends on a block that has not
+// started, whatever order the hardware starts blocks in.  Filing a sphere
+// under a row other than those the tile waited for waits for that row's sort
+// (it reads the bucket and clears the counter the filing increments); that
+// wait is the only one that could be for a row whose block has not started,
+// which with in-order dispatch cannot happen.  Every wait is bounded
+// (wait_limit polls); one that runs out sets *wait_error and its block skips
+// the rest of its work, and the host reports it.
+//
+// Memory ordering: what a row sort writes for the tiles (perm, sorted keys,
+// offsets, the cleared counter and spill count of the set this frame files)
+// is written with agent-scope stores, and the tiles read perm, keys and
+// offsets with agent-scope loads: both are coherent across the XCDs' L2
+// caches without cache maintenance.  Each wave of the row sort waits for its
+// stores (and its reads of the row's bucket) to complete, the block meets at
+// a barrier, and only then is the row's state set; a tile issues those loads
+// only after it has read the state.  (Release and acquire fences would do the
+// same by writing back and invalidating whole caches: a probe of 1024
+// publishing and 4096 reading blocks took 19 us per launch without them,
+// 37 us with the releases and 63 us with both.)
+
+// Tile row words: launch number << kTileRowBits | row (rows < 2^13; the
+// host keeps launch numbers below 2^19 and zeroes the words when it wraps).
+constexpr uint32_t kTileRowBits = 13;
+constexpr uint32_t kTileRowPolls = 256;
+
+// One lane: polls state[r] until it reads `done`; false once `limit` polls
+// have not seen it.
+__device__ __forceinline__ bool wait_row(const uint32_t *state, uint32_t r, uint32_t done,
+                                         uint32_t limit)
+{
+    uint32_t polls = 0;
+    while (__hip_atomic_load(state + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != done) {
+        if (++polls >= limit) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+// may_file for a tile of the one-launch frame: rows lo .. hi were waited for
+// at the tile's start.
+struct RowWait {
+    const uint32_t *state;
+    uint32_t *error;
+    uint32_t done, lo, hi, limit;
+    __device__ __forceinline__ bool operator()(uint32_t r) const
+    {
+        if (r >= lo && r <= hi) return true;
+        if (!wait_row(state, r, done, limit)) {
+            atomicOr(error, 1u);
+            return false;
+        }
+        return true;
+    }
+};
+
+// (at most 72 VGPRs with the small pool, whose LDS fits seven blocks per CU;
+// 80 with the big pool, six)
+template <bool SPRING, uint32_t POOL, bool DENSE, uint32_t CAP>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(POOL == kStagePoolBig ? 6 : 7, 8))) void frame_one_launch_kernel(
+    PhysConsts c, FrameBuffers b, uint32_t set, uint32_t spread_in, uint32_t n, uint32_t tag)
+{
+    extern __shared__ __align__(16) uint32_t s_dyn[];
+    __shared__ uint32_t s_ctl[4];   // claim / rows found / wait result
+    __shared__ uint32_t s_scan[4];
+    // rows this tile claimed in a round of kClaimRound rows, by round
+    // parity: count, then rows (kept small: with the collide pool this
+    // block's LDS stays within 7 blocks per CU)
+    constexpr uint32_t kClaimRound = 16;
+    __shared__ uint32_t s_sort[2][1 + kClaimRound];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t G = c.grids;
+    const uint32_t claimed = 2u * tag + 1u, done = 2u * tag + 2u;
+    uint32_t *state = b.row_flags;
+    // block-wide: sorts row r, whose claim this block holds, and publishes it
+    auto sort_row = [&](uint32_t r) {
+        const uint2 bm = row_sort_body<256, CAP, true>(b.rb, set, spread_in != 0u, b.scratch_pairs,
+                                                       b.scratch_idx, G, n, b.perm, b.off,
+                                                       b.keys_out, r,
+                                                       row_sort_lds_at<256, CAP>(s_dyn, G));
+        // the tiles whose first or last slot lies in this row learn its
+        // number here
+        if (bm.y) {
+            const uint32_t k1 = (bm.x + bm.y - 1u) / 256u;
+            for (uint32_t k = bm.x / 256u + tid; k <= k1; k += 256u) {
+                const uint32_t f = k * 256u, l = min(f + 255u, n - 1u);
+                if (f >= bm.x) st_sc<true>(&b.tile_rows[2u * k], (tag << kTileRowBits) | r);
+                if (l < bm.x + bm.y) st_sc<true>(&b.tile_rows[2u * k + 1u], (tag << kTileRowBits) | r);
+            }
+        }
+        // every wave's stores (and its loads of the row's bucket) complete
+        // before the row is marked sorted
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (tid == 0)
+            __hip_atomic_store(state + r, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    if (blockIdx.x < G) {
+        const uint32_t r = xcd_block(blockIdx.x, G);
+        if (tid == 0) s_ctl[0] = atomicMax(state + r, claimed) < claimed ? 1u : 0u;
+        __syncthreads();
+        if (s_ctl[0]) sort_row(r);
+        return;
+    }
+    // tile `blk`: slots s0 .. s1.  The row sorts that hold slots s0 and s1
+    // publish their row numbers; a tile that has not seen them after
+    // kTileRowPolls polls (only possible if those rows' blocks have not
+    // started) finds them from a scan of this frame's row counts instead.
+    const uint32_t blk = xcd_block(blockIdx.x - G, gridDim.x - G);
+    const uint32_t s0 = blk * 256u;
+    const uint32_t s1 = min(s0 + 255u, n - 1u);
+    if (tid < 2u) {
+        uint32_t v = 0, polls = 0;
+        for (;;) {
+            v = ld_sc<true>(&b.tile_rows[2u * blk + tid]);
+            if ((v >> kTileRowBits) == tag || ++polls >= kTileRowPolls) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+        s_ctl[1u + tid] = (v >> kTileRowBits) == tag ? (v & ((1u << kTileRowBits) - 1u)) : kInvalid;
+        if (tid == 0) s_sort[0][0] = 0u;
+    }
+    __syncthreads();
+    if (s_ctl[1] == kInvalid || s_ctl[2] == kInvalid) {
+        const uint32_t *cnt = spread_in ? b.rb.spread : b.rb.fill + set * G * kFillStride;
+        const uint32_t stride = spread_in ? kSpread : kFillStride;
+        constexpr uint32_t kPer = kMaxRows / 256u;
+        const uint32_t per = (G + 255u) / 256u;
+        const uint32_t r0 = tid * per;
+        uint32_t m[kPer];
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < kPer; ++i) {
+            m[i] = i < per && r0 + i < G ? cnt[(r0 + i) * stride] : 0u;
+            sum += m[i];
+        }
+        uint32_t total;
+        uint32_t run = block_exclusive_scan<256>(sum, s_scan, total);
+        if (tid == 0) {
+            s_ctl[1] = kInvalid;
+            s_ctl[2] = kInvalid;
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t i = 0; i < kPer; ++i) {
+            if (s0 - run < m[i]) s_ctl[1] = r0 + i;  // u32: run <= s0 < run + m[i]
+            if (s1 - run < m[i]) s_ctl[2] = r0 + i;
+            run += m[i];
+        }
+        __syncthreads();
+    }
+    const uint32_t ra = s_ctl[1], rz = s_ctl[2];
+    // (rows not found: the previous frame filed fewer spheres than N, which
+    // only a wait that ran out can cause)
+    const bool found = ra != kInvalid && rz != kInvalid;
+    const uint32_t lo = found && ra > 0u ? ra - 1u : 0u;
+    const uint32_t hi = found ? min(rz + 1u, G - 1u) : 0u;
+    // One pass over lo .. hi, kClaimRound rows a round: a row already sorted
+    // is noted, a row nobody has claimed is claimed and sorted here (the
+    // whole block, at the round's end), a row another block is sorting is
+    // waited for afterwards.  The claim lists alternate by round: a round's
+    // list is cleared after the next round's barrier, when every thread has
+    // read its count.
+    uint32_t seen = 0;  // rounds in which this lane's row was seen sorted (first 32)
+    uint32_t j = 0;
+    for (uint32_t base = lo; found && base <= hi; base += kClaimRound, ++j) {
+        uint32_t *list = s_sort[j & 1u];
+        if (tid < kClaimRound) {
+            const uint32_t r = base + tid;
+            if (r <= hi) {
+                const uint32_t v =
+                    __hip_atomic_load(state + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (v == done)
+                    seen |= j < 32u ? 1u << j : 0u;
+                else if (v < claimed && atomicMax(state + r, claimed) < claimed)
+                    list[1u + atomicAdd(&list[0], 1u)] = r;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) s_sort[(j + 1u) & 1u][0] = 0u;
+        const uint32_t k = list[0];
+        for (uint32_t i = 0; i < k; ++i) sort_row(list[1u + i]);
+    }
+    // then wait for the rows other blocks are sorting
+    if (tid < kClaimRound) {
+        bool ok = found;
+        j = 0;
+        for (uint32_t r = lo + tid; ok && r <= hi; r += kClaimRound, ++j)
+            if (!(j < 32u && ((seen >> j) & 1u))) ok = wait_row(state, r, done, b.wait_limit);
+        const bool all = __ballot(!ok) == 0ull;
+        if (tid == 0) {
+            if (!all) atomicOr(b.wait_error, 1u);
+            s_ctl[3] = all ? 1u : 0u;
+        }
+    }
+    __syncthreads();
+    if (s_ctl[3] == 0u) return;
+    const RowWait may_file{state, b.wait_error, done, lo, hi, b.wait_limit};
+    physics_step_body<SPRING, POOL, false, DENSE, RowWait, true>(
+        c, b.perm, b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, b.rb, set ^ 1u, n, blk,
+        collide_lds_at<POOL, DENSE>(s_dyn), may_file);
+}
+
+template <uint32_t POOL, bool DENSE, uint32_t CAP>
+static void launch_one_launch_cap(const FrameBuffers &b, const PhysConsts &c, uint32_t set,
+                                  uint32_t spread_in, uint32_t n, uint32_t tag,
+                                  hipStream_t stream)
+{
+    const uint32_t G = c.grids;
+    const uint32_t blocks = G + (n + 255u) / 256u;
+    const size_t lds = std::max<size_t>((size_t)row_sort_lds_words<256, CAP>(G) * 4u,
+                                        collide_lds_bytes<POOL, DENSE>());
+    if (c.enable_spring)
+        hipLaunchKernelGGL((frame_one_launch_kernel<true, POOL, DENSE, CAP>), dim3(blocks),
+                           dim3(256), lds, stream, c, b, set, spread_in, n, tag);
+    else
+        hipLaunchKernelGGL((frame_one_launch_kernel<false, POOL, DENSE, CAP>), dim3(blocks),
+                           dim3(256), lds, stream, c, b, set, spread_in, n, tag);
+}
+
+// Rows of up to kRowCapNarrow spheres are sorted in LDS with the small
+// staging pool, whose seven blocks per CU leave 72 VGPRs a lane (a longer
+// row is sorted in global memory); up to kRowCap with the big pool, whose
+// six blocks per CU leave 80.
+template <uint32_t POOL, bool DENSE>
+static void launch_one_launch(const FrameBuffers &b, const PhysConsts &c, uint32_t set,
+                              uint32_t spread_in, uint32_t n, uint32_t tag, hipStream_t stream)
+{
+    launch_one_launch_cap<POOL, DENSE, POOL == kStagePoolBig ? kRowCap : kRowCapNarrow>(
+        b, c, set, spread_in, n, tag, stream);
+}
+
 hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time, uint32_t n,
                         hipStream_t stream, hipEvent_t *events, FrameTuning tuning)
 {
@@ -1881,6 +2245,23 @@ hipError_t launch_frame(const FrameBuffers &b, const PhysConsts &c, int64_t time
         return hipGetLastError();
     }
     const uint32_t set = (uint32_t)(time & 1);
+    if (tuning.one_launch) {
+        if ((e = mark(1)) != hipSuccess) return e;
+        const uint32_t spread_in = time == 1 ? 1u : 0u;
+        const uint32_t tag = b.launch;
+        if (tuning.big_pool) {
+            if (tuning.dense_scan)
+                launch_one_launch<kStagePoolBig, true>(b, c, set, spread_in, n, tag, stream);
+            else
+                launch_one_launch<kStagePoolBig, false>(b, c, set, spread_in, n, tag, stream);
+        } else if (tuning.dense_scan) {
+            launch_one_launch<kStagePool, true>(b, c, set, spread_in, n, tag, stream);
+        } else {
+            launch_one_launch<kStagePool, false>(b, c, set, spread_in, n, tag, stream);
+        }
+        if ((e = mark(2)) != hipSuccess) return e;
+        return hipGetLastError();
+    }
     if (tuning.narrow_sort)
         hipLaunchKernelGGL((row_sort_kernel<256, kRowCapNarrow>), dim3(G), dim3(256),
                            (row_sort_lds<256, kRowCapNarrow>(G)), stream, b.rb, set,
@@ -1935,6 +2316,7 @@ __device__ __forceinline__ RowBuckets scene_buckets(const SceneDesc &S)
     rb.spill_count = gptr(S.rb.spill_count);
     rb.spread = gptr(S.rb.spread);
     rb.cap = S.rb.cap;
+    rb.spill_stride = S.rb.spill_stride;
     return rb;
 }
 
@@ -1974,8 +2356,10 @@ __global__ __launch_bounds__(NT) void batch_row_sort_kernel(const SceneDesc *__r
         }
         return;
     }
+    extern __shared__ __align__(16) uint32_t s_dyn[];
     row_sort_body<NT, CAP>(rb, (uint32_t)time & 1u, time == 1, gptr(S.scratch_pairs),
-                      gptr(S.scratch_idx), G, S.n, gptr(S.perm), off, gptr(S.keys), m.y);
+                           gptr(S.scratch_idx), G, S.n, gptr(S.perm), off, gptr(S.keys), m.y,
+                           row_sort_lds_at<NT, CAP>(s_dyn, G));
 }
 
 // DENSE: the scenes whose submitted state crowds its cells, on the dense
@@ -1986,9 +2370,7 @@ __global__ __launch_bounds__(256) void batch_physics_step_kernel(
     const uint32_t *__restrict__ counter)
 {
     constexpr uint32_t POOL = DENSE ? kStagePoolBig : kStagePool;
-    __shared__ float4 s_pos[POOL];
-    __shared__ uint2 s_vm[POOL];
-    __shared__ uint32_t s_slab[DENSE ? kSlabs * slab_words(POOL) : 1u];
+    __shared__ __align__(16) char s_lds[collide_lds_bytes<POOL, DENSE>()];
     const uint2 m = map[xcd_block(blockIdx.x, gridDim.x)];
     const SceneDesc &S = scenes[m.x];
     const int32_t time = scene_time(S, plan_frame(k, counter));
@@ -2002,7 +2384,8 @@ __global__ __launch_bounds__(256) void batch_physics_step_kernel(
     const uint32_t p = (uint32_t)time & 1u;
     physics_step_body<SPRING, POOL, false, DENSE>(
         S.c, gptr(S.perm), gptr(S.off), gptr(S.keys), gptr(S.pos[p]), gptr(S.vfme[p]),
-        gptr(S.pos[p ^ 1u]), gptr(S.vfme[p ^ 1u]), rb, p ^ 1u, S.n, m.y, s_pos, s_vm, s_slab);
+        gptr(S.pos[p ^ 1u]), gptr(S.vfme[p ^ 1u]), rb, p ^ 1u, S.n, m.y,
+        collide_lds_at<POOL, DENSE>(s_lds));
 }
 
 __global__ __launch_bounds__(64) void batch_advance_kernel(uint32_t *__restrict__ counter)

@@ -247,6 +247,7 @@ struct Request {
         mb::bucket_counters(reinterpret_cast<uint32_t *>(region + layout.counters), (uint32_t)g,
                             d.rb);
         d.rb.cap = layout.cap;
+        d.rb.spill_stride = 0;  // separate kernels per frame: one list serves both sets
         d.scratch_pairs = reinterpret_cast<uint64_t *>(region + layout.scratch_pairs);
         d.scratch_idx = reinterpret_cast<uint32_t *>(region + layout.scratch);
         return d;

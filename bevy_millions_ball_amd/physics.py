@@ -229,19 +229,26 @@ class Physics:
 
     # -- tuning (launch choices that do not change results) ---------------
     @property
-    def tuning(self) -> Tuple[float, int, int, int]:
-        """(scan_margin, stage_pool, row_sort_threads, dense_scan)."""
+    def tuning(self) -> Tuple[float, int, int, int, int]:
+        """(scan_margin, stage_pool, row_sort_threads, dense_scan, one_launch)."""
         t = MbTuning()
         check(lib().mb_physics_get_tuning(self._h, C.byref(t)))
-        return float(t.scan_margin), int(t.stage_pool), int(t.row_sort_threads), int(t.dense_scan)
+        return (float(t.scan_margin), int(t.stage_pool), int(t.row_sort_threads),
+                int(t.dense_scan), int(t.one_launch))
 
     def set_tuning(self, scan_margin: float, stage_pool: int, row_sort_threads: int = 512,
-                   dense_scan: int = 0) -> None:
+                   dense_scan: int = 0, one_launch: int = 0) -> None:
         t = MbTuning()
         check(lib().mb_physics_get_tuning(self._h, C.byref(t)))
         t.scan_margin, t.stage_pool = float(scan_margin), int(stage_pool)
         t.row_sort_threads, t.dense_scan = int(row_sort_threads), int(dense_scan)
+        t.one_launch = int(one_launch)
         check(lib().mb_physics_set_tuning(self._h, C.byref(t)))
+
+    def set_wait_limit(self, polls: int) -> None:
+        """Polls of one bounded wait of the one-launch frame (0: the default);
+        for tests of its failure path."""
+        check(lib().mb_physics_debug_set_wait_limit(self._h, int(polls)))
 
     @property
     def frame_graph(self) -> bool:
@@ -256,15 +263,15 @@ class Physics:
         t.frame_graph = int(bool(enable))
         check(lib().mb_physics_set_tuning(self._h, C.byref(t)))
 
-    def autotune(self, frames_per_trial: int = 5) -> Tuple[Tuple[float, int, int, int], float]:
+    def autotune(self, frames_per_trial: int = 5) -> Tuple[Tuple[float, int, int, int, int], float]:
         """Times every candidate tuning on a copy of the state, keeps the
         fastest; returns ((scan_margin, stage_pool, row_sort_threads,
-        dense_scan), ms per frame)."""
+        dense_scan, one_launch), ms per frame)."""
         t = MbTuning()
         ms = C.c_double()
         check(lib().mb_physics_autotune(self._h, int(frames_per_trial), C.byref(t), C.byref(ms)))
         return (float(t.scan_margin), int(t.stage_pool), int(t.row_sort_threads),
-                int(t.dense_scan)), ms.value
+                int(t.dense_scan), int(t.one_launch)), ms.value
 
     # -- checkpoints --------------------------------------------------------
     def save_checkpoint(self) -> bytes:

@@ -143,21 +143,33 @@ int mb_physics_download_state(const mb_physics *physics, float *position_with_sc
  * spheres sorted in LDS) or 256 (up to 2048; twice as many rows resident);
  * frame_graph (0 / 1) lets mb_physics_step replay runs of LOOP frames as a
  * captured hipGraph of two frames instead of launching each kernel (never
- * while profiling; the frame after INIT and frames on the stage kernels are
- * launched directly); dense_scan (0 / 1) runs the collide variant in which
+ * while profiling; the frame after INIT, frames on the stage kernels and
+ * one-launch frames are launched directly); dense_scan (0 / 1) runs the collide variant in which
  * waves with a lane of more than 48 candidates scan them range by range,
  * with one reach from the block's largest radius, testing only the
  * candidates whose height slab meets the lane's reach (far fewer tests where
- * cells are crowded, a little slower near one sphere per cell).
- * Defaults: 1.0, 896, 512, 0 and 0.  The autotuner searches scan_margin
- * (1, 1/2, 1/4, 1/16) x stage_pool x row_sort_threads x dense_scan, then
- * times its winner with the other frame_graph choice. */
+ * cells are crowded, a little slower near one sphere per cell);
+ * one_launch (0 / 1) runs each LOOP frame as one kernel launch instead of
+ * two (the reference's three dispatch groups, mod.rs:846-912): its first G
+ * blocks sort the rows, the others run the collide tiles, each starting as
+ * soon as the rows around its slots are sorted (row sort blocks of 256
+ * threads: row_sort_threads does not apply; rows of up to 2048 spheres, 4096
+ * with the 1408-slot pool, are sorted in LDS).  A tile that finds a row it
+ * needs not yet started sorts that row itself, so no wait depends on the
+ * order in which blocks start.  Its waits are still bounded; a wait that
+ * runs out (never expected) makes the next call that synchronises
+ * (downloads, checkpoints, debug views, profile reads, autotune) fail with
+ * MB_ERR_STATE, the state being invalid.
+ * Defaults: 1.0, 896, 512, 0, 0 and 0.  The autotuner searches scan_margin
+ * (1, 1/2, 1/4, 1/16) x stage_pool x row_sort_threads x dense_scan x
+ * one_launch, then times its winner with the other frame_graph choice. */
 typedef struct mb_tuning {
     float scan_margin;
     uint32_t stage_pool;
     uint32_t row_sort_threads;
     uint32_t frame_graph;
     uint32_t dense_scan;
+    uint32_t one_launch;
 } mb_tuning;
 
 int mb_physics_get_tuning(const mb_physics *physics, mb_tuning *tuning);
@@ -219,12 +231,20 @@ int mb_physics_debug_download(const mb_physics *physics, uint32_t *perm,
  * [0] sort (the row sort, which also writes the cell ranges of spatial
  * hashing), [1] collide + integrate (which also files every sphere in the
  * row bucket of its next cell for the next frame's sort).  Recording does
- * not block.  mb_physics_profile_read waits for the recorded frames, writes
+ * not block (a one-launch frame has one kernel: its time is all in [1]).
+ * mb_physics_profile_read waits for the recorded frames, writes
  * up to `capacity` rows of two milliseconds values to stage_ms (may be NULL
  * to only count), stores the row count in *frames and forgets them. */
 int mb_physics_set_profiling(mb_physics *physics, int enable);
 int mb_physics_profile_read(mb_physics *physics, double *stage_ms, uint32_t capacity,
                             uint32_t *frames);
+
+/* No counterpart in the reference; for tests of the one-launch frame's
+ * failure path.  Sets how many times one of its waits polls before it gives
+ * up (0: the default, 2^22, about a tenth of a second).  A limit of 1 makes
+ * tiles that find their rows unsorted give up at once, so the frame reports
+ * MB_ERR_STATE without any wait that could hang. */
+int mb_physics_debug_set_wait_limit(mb_physics *physics, uint32_t polls);
 
 const char *mb_last_error(void);
 

@@ -59,6 +59,7 @@ pub struct mb_tuning {
     pub row_sort_threads: u32,
     pub frame_graph: u32,
     pub dense_scan: u32,
+    pub one_launch: u32,
 }
 
 #[repr(C)]
@@ -127,6 +128,7 @@ extern "C" {
     pub fn mb_physics_set_profiling(physics: *mut mb_physics, enable: c_int) -> c_int;
     pub fn mb_physics_profile_read(physics: *mut mb_physics, stage_ms: *mut f64, capacity: u32,
                                    frames: *mut u32) -> c_int;
+    pub fn mb_physics_debug_set_wait_limit(physics: *mut mb_physics, polls: u32) -> c_int;
     pub fn mb_last_error() -> *const c_char;
 
     pub fn mb_radix_sort_workspace_size(max_number_of_keys: u32) -> usize;

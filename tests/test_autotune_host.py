@@ -29,8 +29,11 @@ def test_cache_round_trip(tmp_path):
     path = tmp_path / "t.json"
     c = TuningCache(path)
     assert c.get("k") is None
-    c.put("k", (0.5, 1408, 256, 1), 0.123)
-    assert TuningCache(path).get("k") == (0.5, 1408, 256, 1)
+    c.put("k", (0.5, 1408, 256, 1, 1), 0.123)
+    assert TuningCache(path).get("k") == (0.5, 1408, 256, 1, 1)
+    # an entry written before the one-launch frame existed reads as one_launch 0
+    path.write_text('{"old": {"scan_margin": 1.0, "stage_pool": 896, "ms_per_frame": 0.05}}')
+    assert TuningCache(path).get("old") == (1.0, 896, 512, 0, 0)
     path.write_text("not json")
     assert TuningCache(path).get("k") is None  # a damaged cache is ignored
 
@@ -47,12 +50,12 @@ class FakePhysics:
     def download_state(self):
         return self._pos, None
 
-    def set_tuning(self, margin, pool, sort=512, dense=0):
-        self.applied = (margin, pool, sort, dense)
+    def set_tuning(self, margin, pool, sort=512, dense=0, one=0):
+        self.applied = (margin, pool, sort, dense, one)
 
     def autotune(self, frames):
         self.autotuned += 1
-        self.applied = (0.25, 1408, 256, 1)
+        self.applied = (0.25, 1408, 256, 1, 1)
         return self.applied, 1.5
 
 
@@ -61,7 +64,7 @@ def test_tune_times_once_then_uses_the_cache(tmp_path):
     pos, _ = generate(cfg, seed=0)
     cache = TuningCache(tmp_path / "t.json")
     first = FakePhysics(cfg.settings(), pos)
-    assert tune(first, cache) == ((0.25, 1408, 256, 1), False) and first.autotuned == 1
+    assert tune(first, cache) == ((0.25, 1408, 256, 1, 1), False) and first.autotuned == 1
     second = FakePhysics(cfg.settings(), pos)
-    assert tune(second, TuningCache(tmp_path / "t.json")) == ((0.25, 1408, 256, 1), True)
-    assert second.autotuned == 0 and second.applied == (0.25, 1408, 256, 1)
+    assert tune(second, TuningCache(tmp_path / "t.json")) == ((0.25, 1408, 256, 1, 1), True)
+    assert second.autotuned == 0 and second.applied == (0.25, 1408, 256, 1, 1)
