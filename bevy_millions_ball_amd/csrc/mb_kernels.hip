@@ -1699,22 +1699,26 @@ struct CollideLds {
     float *wave_rmax;
 };
 
-template <uint32_t POOL, bool DENSE>
+// The spring response reads no neighbour velocity or mass, so spring blocks
+// stage positions only (16 instead of 24 bytes a slot: at the 1408-slot pool
+// with the slab bitmaps, six blocks per CU instead of four).
+template <uint32_t POOL, bool DENSE, bool SPRING>
 __host__ __device__ constexpr uint32_t collide_lds_bytes()
 {
-    return POOL * 24u + (DENSE ? kSlabs * slab_words(POOL) * 4u : 0u) +
+    return POOL * (SPRING ? 16u : 24u) + (DENSE ? kSlabs * slab_words(POOL) * 4u : 0u) +
            (uint32_t)sizeof(BlockFiling) + 16u;
 }
 
-template <uint32_t POOL, bool DENSE>
+template <uint32_t POOL, bool DENSE, bool SPRING>
 __device__ __forceinline__ CollideLds collide_lds_at(void *base)
 {
+    constexpr uint32_t kStage = POOL * (SPRING ? 16u : 24u);
     CollideLds l;
     char *p = static_cast<char *>(base);
     l.pos = reinterpret_cast<float4 *>(p);
-    l.vm = reinterpret_cast<uint2 *>(p + POOL * 16u);
-    l.slab = reinterpret_cast<uint32_t *>(p + POOL * 24u);
-    l.file = reinterpret_cast<BlockFiling *>(p + POOL * 24u +
+    l.vm = SPRING ? nullptr : reinterpret_cast<uint2 *>(p + POOL * 16u);
+    l.slab = reinterpret_cast<uint32_t *>(p + kStage);
+    l.file = reinterpret_cast<BlockFiling *>(p + kStage +
                                              (DENSE ? kSlabs * slab_words(POOL) * 4u : 0u));
     l.wave_rmax = reinterpret_cast<float *>(l.file + 1);
     return l;
@@ -1862,10 +1866,10 @@ __global__ __launch_bounds__(256) void physics_step_kernel(
     const uint4 *__restrict__ vfme_i, float4 *__restrict__ pos_o, uint4 *__restrict__ vfme_o,
     RowBuckets rb, uint32_t set_out, uint32_t n)
 {
-    __shared__ __align__(16) char s_lds[collide_lds_bytes<POOL, DENSE>()];
+    __shared__ __align__(16) char s_lds[collide_lds_bytes<POOL, DENSE, SPRING>()];
     physics_step_body<SPRING, POOL, PRIO, DENSE>(c, perm, off, keys, pos_i, vfme_i, pos_o, vfme_o,
                                                  rb, set_out, n, xcd_block(blockIdx.x, gridDim.x),
-                                                 collide_lds_at<POOL, DENSE>(s_lds));
+                                                 collide_lds_at<POOL, DENSE, SPRING>(s_lds));
 }
 
 // INIT frame of the frame path: the sort of the all-zero keys is the
@@ -2191,7 +2195,7 @@ __attribute__((amdgpu_waves_per_eu(POOL == kStagePoolBig ? 6 : 7, 8))) void fram
     const RowWait may_file{state, b.wait_error, done, lo, hi, b.wait_limit};
     physics_step_body<SPRING, POOL, false, DENSE, RowWait, true>(
         c, b.perm, b.off, b.keys_out, b.pos_i, b.vfme_i, b.pos_o, b.vfme_o, b.rb, set ^ 1u, n, blk,
-        collide_lds_at<POOL, DENSE>(s_dyn), may_file);
+        collide_lds_at<POOL, DENSE, SPRING>(s_dyn), may_file);
 }
 
 template <uint32_t POOL, bool DENSE, uint32_t CAP>
@@ -2201,14 +2205,15 @@ static void launch_one_launch_cap(const FrameBuffers &b, const PhysConsts &c, ui
 {
     const uint32_t G = c.grids;
     const uint32_t blocks = G + (n + 255u) / 256u;
-    const size_t lds = std::max<size_t>((size_t)row_sort_lds_words<256, CAP>(G) * 4u,
-                                        collide_lds_bytes<POOL, DENSE>());
+    const size_t rows = (size_t)row_sort_lds_words<256, CAP>(G) * 4u;
     if (c.enable_spring)
         hipLaunchKernelGGL((frame_one_launch_kernel<true, POOL, DENSE, CAP>), dim3(blocks),
-                           dim3(256), lds, stream, c, b, set, spread_in, n, tag);
+                           dim3(256), std::max<size_t>(rows, collide_lds_bytes<POOL, DENSE, true>()),
+                           stream, c, b, set, spread_in, n, tag);
     else
         hipLaunchKernelGGL((frame_one_launch_kernel<false, POOL, DENSE, CAP>), dim3(blocks),
-                           dim3(256), lds, stream, c, b, set, spread_in, n, tag);
+                           dim3(256), std::max<size_t>(rows, collide_lds_bytes<POOL, DENSE, false>()),
+                           stream, c, b, set, spread_in, n, tag);
 }
 
 // Rows of up to kRowCapNarrow spheres are sorted in LDS with the small
@@ -2370,7 +2375,7 @@ __global__ __launch_bounds__(256) void batch_physics_step_kernel(
     const uint32_t *__restrict__ counter)
 {
     constexpr uint32_t POOL = DENSE ? kStagePoolBig : kStagePool;
-    __shared__ __align__(16) char s_lds[collide_lds_bytes<POOL, DENSE>()];
+    __shared__ __align__(16) char s_lds[collide_lds_bytes<POOL, DENSE, SPRING>()];
     const uint2 m = map[xcd_block(blockIdx.x, gridDim.x)];
     const SceneDesc &S = scenes[m.x];
     const int32_t time = scene_time(S, plan_frame(k, counter));
@@ -2385,7 +2390,7 @@ __global__ __launch_bounds__(256) void batch_physics_step_kernel(
     physics_step_body<SPRING, POOL, false, DENSE>(
         S.c, gptr(S.perm), gptr(S.off), gptr(S.keys), gptr(S.pos[p]), gptr(S.vfme[p]),
         gptr(S.pos[p ^ 1u]), gptr(S.vfme[p ^ 1u]), rb, p ^ 1u, S.n, m.y,
-        collide_lds_at<POOL, DENSE>(s_lds));
+        collide_lds_at<POOL, DENSE, SPRING>(s_lds));
 }
 
 __global__ __launch_bounds__(64) void batch_advance_kernel(uint32_t *__restrict__ counter)
