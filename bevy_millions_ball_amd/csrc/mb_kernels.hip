@@ -1481,7 +1481,7 @@ template <bool SPRING, bool FROM_LDS, bool SC = false>
 __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, uint32_t s,
                                                 const Ranges r,
                                                 const float4 *__restrict__ s_pos,
-                                                const uint2 *__restrict__ s_vm,
+                                                const uint32_t *__restrict__ s_src,
                                                 const uint32_t *__restrict__ perm,
                                                 const float4 *__restrict__ pos_i,
                                                 const uint4 *__restrict__ vfme_i)
@@ -1538,7 +1538,8 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
             bool moved;
             if (FROM_LDS) {
                 const uint32_t l = lds_of(t);
-                moved = collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
+                moved = collide_candidate<SPRING>(
+                    c, a, s_pos[l], [&]() { return vel_mass_of(vfme_i[s_src[l]]); });
             } else {
                 const uint32_t bo = ld_sc<SC>(&perm[slot_of(t)]);
                 moved = collide_candidate<SPRING>(c, a, pos_i[bo],
@@ -1594,7 +1595,8 @@ template <bool SPRING>
 __device__ __forceinline__ void walk_dense(const PhysConsts &c, Agent &a, uint32_t s,
                                            const Ranges r, float rmax,
                                            const float4 *__restrict__ s_pos,
-                                           const uint2 *__restrict__ s_vm,
+                                           const uint32_t *__restrict__ s_src,
+                                           const uint4 *__restrict__ vfme_i,
                                            const uint32_t *__restrict__ s_slab, uint32_t sw,
                                            float inv_h)
 {
@@ -1663,7 +1665,8 @@ __device__ __forceinline__ void walk_dense(const PhysConsts &c, Agent &a, uint32
             mask[2] = (!h0 && !h1 && h2) ? rest : mask[2];
             mask[3] = (!h0 && !h1 && !h2) ? rest : mask[3];
             const uint32_t l = p0 + i;
-            const bool moved = collide_candidate<SPRING>(c, a, s_pos[l], [&]() { return s_vm[l]; });
+            const bool moved = collide_candidate<SPRING>(
+                c, a, s_pos[l], [&]() { return vel_mass_of(vfme_i[s_src[l]]); });
             if (moved) {
                 const float ex = a.px - rx, ey = a.py - ry, ez = a.pz - rz;
                 if (!(ex * ex + ey * ey + ez * ez <= still)) {
@@ -1693,30 +1696,35 @@ __device__ __forceinline__ void walk_dense(const PhysConsts &c, Agent &a, uint32
 // filing tables and the waves' largest radii.
 struct CollideLds {
     float4 *pos;
-    uint2 *vm;
+    uint32_t *src;   // the staged neighbours' indices into the input state (rigid blocks)
     uint32_t *slab;
     BlockFiling *file;
     float *wave_rmax;
 };
 
-// The spring response reads no neighbour velocity or mass, so spring blocks
-// stage positions only (16 instead of 24 bytes a slot: at the 1408-slot pool
-// with the slab bitmaps, six blocks per CU instead of four).
+// A block stages each neighbour's position (16 bytes) and, for the rigid
+// response, its index into the input state (4 bytes): a contact, about 0.3
+// per sphere at one sphere per cell, reads the neighbour's velocity and mass
+// from global memory through it.  Staging the velocity and mass halves
+// instead (8 bytes a slot) kept seven blocks per CU at the 896-slot pool;
+// the index keeps eight (1M spheres: 48.1 -> 46.2 us per frame, 4M: 183 ->
+// 173 us; 65K, whose 256 blocks never fill the chip, 14.7 -> 15.2 us).  The
+// spring response reads neither, so spring blocks stage positions only.
 template <uint32_t POOL, bool DENSE, bool SPRING>
 __host__ __device__ constexpr uint32_t collide_lds_bytes()
 {
-    return POOL * (SPRING ? 16u : 24u) + (DENSE ? kSlabs * slab_words(POOL) * 4u : 0u) +
+    return POOL * (SPRING ? 16u : 20u) + (DENSE ? kSlabs * slab_words(POOL) * 4u : 0u) +
            (uint32_t)sizeof(BlockFiling) + 16u;
 }
 
 template <uint32_t POOL, bool DENSE, bool SPRING>
 __device__ __forceinline__ CollideLds collide_lds_at(void *base)
 {
-    constexpr uint32_t kStage = POOL * (SPRING ? 16u : 24u);
+    constexpr uint32_t kStage = POOL * (SPRING ? 16u : 20u);
     CollideLds l;
     char *p = static_cast<char *>(base);
     l.pos = reinterpret_cast<float4 *>(p);
-    l.vm = SPRING ? nullptr : reinterpret_cast<uint2 *>(p + POOL * 16u);
+    l.src = SPRING ? nullptr : reinterpret_cast<uint32_t *>(p + POOL * 16u);
     l.slab = reinterpret_cast<uint32_t *>(p + kStage);
     l.file = reinterpret_cast<BlockFiling *>(p + kStage +
                                              (DENSE ? kSlabs * slab_words(POOL) * 4u : 0u));
@@ -1734,7 +1742,7 @@ __device__ __forceinline__ void physics_step_body(
     const MayFile &may_file = MayFile())
 {
     float4 *__restrict__ s_pos = lds.pos;
-    uint2 *__restrict__ s_vm = lds.vm;
+    uint32_t *__restrict__ s_src = lds.src;
     uint32_t *__restrict__ s_slab = lds.slab;
     BlockFiling &s_file = *lds.file;
     float *s_wave_rmax = lds.wave_rmax;
@@ -1818,7 +1826,7 @@ __device__ __forceinline__ void physics_step_body(
                 rmax = fmaxf(rmax, bp.w);
                 atomicOr(&s_slab[slab_of(bp.y, inv_h) * sw + (j >> 5)], 1u << (j & 31u));
             }
-            if (!SPRING) s_vm[j] = vel_mass_of(vfme_i[bo]);
+            if (!SPRING) s_src[j] = bo;
         }
         if (DENSE) {
             // the largest radius in the windows, for the dense scan's reach
@@ -1845,11 +1853,11 @@ __device__ __forceinline__ void physics_step_body(
     if (DENSE && split) {
         const float rmax = fmaxf(fmaxf(s_wave_rmax[0], s_wave_rmax[1]),
                                  fmaxf(s_wave_rmax[2], s_wave_rmax[3]));
-        walk_dense<SPRING>(c, a, s, rg, rmax, s_pos, s_vm, s_slab, sw, inv_h);
+        walk_dense<SPRING>(c, a, s, rg, rmax, s_pos, s_src, vfme_i, s_slab, sw, inv_h);
     } else if (staged) {
-        walk_candidates<SPRING, true>(c, a, s, rg, s_pos, s_vm, perm, pos_i, vfme_i);
+        walk_candidates<SPRING, true>(c, a, s, rg, s_pos, s_src, perm, pos_i, vfme_i);
     } else {
-        walk_candidates<SPRING, false, SC>(c, a, s, rg, s_pos, s_vm, perm, pos_i, vfme_i);
+        walk_candidates<SPRING, false, SC>(c, a, s, rg, s_pos, s_src, perm, pos_i, vfme_i);
     }
     jitter(a, s, ox, oy, oz);
     const uint32_t next = finish_agent<SPRING, true>(c, a, s, pos_o, vfme_o, keys);
