@@ -2059,8 +2059,8 @@ struct RowWait {
     }
 };
 
-// (at most 72 VGPRs with the small pool, whose LDS fits seven blocks per CU;
-// 80 with the big pool, six)
+// (at most 72 VGPRs with the small pool, seven blocks per CU: 64 would spill
+// 44 bytes a lane; 80 with the big pool, six)
 template <bool SPRING, uint32_t POOL, bool DENSE, uint32_t CAP>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(POOL == kStagePoolBig ? 6 : 7, 8))) void frame_one_launch_kernel(
@@ -2070,8 +2070,8 @@ __attribute__((amdgpu_waves_per_eu(POOL == kStagePoolBig ? 6 : 7, 8))) void fram
     __shared__ uint32_t s_ctl[4];   // claim / rows found / wait result
     __shared__ uint32_t s_scan[4];
     // rows this tile claimed in a round of kClaimRound rows, by round
-    // parity: count, then rows (kept small: with the collide pool this
-    // block's LDS stays within 7 blocks per CU)
+    // parity: count, then rows (kept small, so that the collide pool and
+    // this stay within one LDS allocation granule of the two-launch collide)
     constexpr uint32_t kClaimRound = 16;
     __shared__ uint32_t s_sort[2][1 + kClaimRound];
     const uint32_t tid = threadIdx.x;
@@ -2225,9 +2225,9 @@ static void launch_one_launch_cap(const FrameBuffers &b, const PhysConsts &c, ui
 }
 
 // Rows of up to kRowCapNarrow spheres are sorted in LDS with the small
-// staging pool, whose seven blocks per CU leave 72 VGPRs a lane (a longer
-// row is sorted in global memory); up to kRowCap with the big pool, whose
-// six blocks per CU leave 80.
+// staging pool, at seven blocks per CU and 72 VGPRs a lane (a longer row is
+// sorted in global memory); up to kRowCap with the big pool, at six blocks
+// per CU and 80.
 template <uint32_t POOL, bool DENSE>
 static void launch_one_launch(const FrameBuffers &b, const PhysConsts &c, uint32_t set,
                               uint32_t spread_in, uint32_t n, uint32_t tag, hipStream_t stream)
