@@ -1404,7 +1404,16 @@ static size_t row_sort_lds(uint32_t grids)
 // solve_sphere_constraint behind a cheap conservative filter:
 // sqrt_rn(d2) <= edge implies d2 <= edge^2 (1 + 2^-23), far inside the 1e-4
 // margin used here, so every candidate that can be in contact reaches the
-// exact test.
+// exact test.  contact_filter is that filter alone, with the same arithmetic.
+__device__ __forceinline__ bool contact_filter(const Agent &a, float4 bp)
+{
+    const float edge = bp.w + a.scale;
+    const float tx = a.px - bp.x;
+    const float ty = a.py - bp.y;
+    const float tz = a.pz - bp.z;
+    return tx * tx + ty * ty + tz * tz <= (edge * edge) * 1.0001f;
+}
+
 template <bool SPRING, typename VelMass>
 __device__ __forceinline__ bool collide_candidate(const PhysConsts &c, Agent &a, float4 bp,
                                                   VelMass vel_mass)
@@ -1532,19 +1541,35 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
         if (t_self >= j && t_self < end) near &= ~(1u << (end - 1u - t_self));
         uint32_t next = end;
         while (near) {
-            const uint32_t lz = (uint32_t)__builtin_clz(near);
-            const uint32_t t = end - 32u + lz;
-            near ^= 0x80000000u >> lz;
+            // Each lane first passes over its near candidates that fail the
+            // contact filter (they change nothing) up to the next that
+            // passes it; then the lanes that found one run the contact
+            // response together, so a wave runs it about as often as its
+            // busiest lane has contacts rather than once for every candidate
+            // position at which some lane has one.
+            uint32_t t, l;
+            float4 bp;
+            bool hit;
+            do {
+                const uint32_t lz = (uint32_t)__builtin_clz(near);
+                t = end - 32u + lz;
+                near ^= 0x80000000u >> lz;
+                if (FROM_LDS) {
+                    l = lds_of(t);
+                    bp = s_pos[l];
+                } else {
+                    l = ld_sc<SC>(&perm[slot_of(t)]);
+                    bp = pos_i[l];
+                }
+                hit = contact_filter(a, bp);
+            } while (!hit && near);
+            if (!hit) break;
             bool moved;
-            if (FROM_LDS) {
-                const uint32_t l = lds_of(t);
-                moved = collide_candidate<SPRING>(
-                    c, a, s_pos[l], [&]() { return vel_mass_of(vfme_i[s_src[l]]); });
-            } else {
-                const uint32_t bo = ld_sc<SC>(&perm[slot_of(t)]);
-                moved = collide_candidate<SPRING>(c, a, pos_i[bo],
-                                                  [&]() { return vel_mass_of(vfme_i[bo]); });
-            }
+            uint2 q = make_uint2(0u, 0u);
+            // the neighbour's velocity and mass, read before the contact's
+            // square root and quotients so that its latency overlaps them
+            if (!SPRING) q = vel_mass_of(vfme_i[FROM_LDS ? s_src[l] : l]);
+            moved = collide_candidate<SPRING>(c, a, bp, [&]() { return q; });
             if (moved) {
                 const float ex = a.px - rx, ey = a.py - ry, ez = a.pz - rz;
                 if (!(ex * ex + ey * ey + ez * ez <= still)) {
@@ -1817,16 +1842,40 @@ __device__ __forceinline__ void physics_step_body(
         const uint32_t d1 = (wlo[1] - wbase[1]) - wlo[0];
         const uint32_t d2 = (wlo[2] - wbase[2]) - (wlo[1] - wbase[1]);
         float rmax = -INFINITY;
-        for (uint32_t j = tid; j < wtot; j += 256u) {
-            const uint32_t bo = ld_sc<SC>(
-                &perm[j + wlo[0] + (j >= wbase[1] ? d1 : 0u) + (j >= wbase[2] ? d2 : 0u)]);
-            const float4 bp = pos_i[bo];
-            s_pos[j] = bp;
-            if (DENSE) {
-                rmax = fmaxf(rmax, bp.w);
-                atomicOr(&s_slab[slab_of(bp.y, inv_h) * sw + (j >> 5)], 1u << (j & 31u));
+        // up to four slots a thread at a time: their input indices are read
+        // together, then their positions, so that a thread waits for two
+        // round trips to memory per four slots instead of two per slot
+        constexpr uint32_t kBatch = 4;
+        for (uint32_t j0 = 0; j0 < wtot; j0 += 256u * kBatch) {
+            uint32_t bo[kBatch];
+            float4 bp[kBatch];
+#pragma unroll
+            for (uint32_t i = 0; i < kBatch; ++i) {
+                const uint32_t j = j0 + tid + 256u * i;
+                bo[i] = j < wtot ? ld_sc<SC>(&perm[j + wlo[0] + (j >= wbase[1] ? d1 : 0u) +
+                                                   (j >= wbase[2] ? d2 : 0u)])
+                                 : 0u;
             }
-            if (!SPRING) s_src[j] = bo;
+#pragma unroll
+            for (uint32_t i = 0; i < kBatch; ++i) bp[i] = pos_i[bo[i]];  // slot 0's beyond wtot
+            // all of them in flight before the first is used (keeps the
+            // compiler from moving each read into its store's branch)
+#pragma unroll
+            for (uint32_t i = 0; i < kBatch; ++i)
+                asm volatile("" ::"v"(bp[i].x), "v"(bp[i].y), "v"(bp[i].z), "v"(bp[i].w));
+#pragma unroll
+            for (uint32_t i = 0; i < kBatch; ++i) {
+                const uint32_t j = j0 + tid + 256u * i;
+                if (j < wtot) {
+                    s_pos[j] = bp[i];
+                    if (DENSE) {
+                        rmax = fmaxf(rmax, bp[i].w);
+                        atomicOr(&s_slab[slab_of(bp[i].y, inv_h) * sw + (j >> 5)],
+                                 1u << (j & 31u));
+                    }
+                    if (!SPRING) s_src[j] = bo[i];
+                }
+            }
         }
         if (DENSE) {
             // the largest radius in the windows, for the dense scan's reach
