@@ -1541,35 +1541,50 @@ __device__ __forceinline__ void walk_candidates(const PhysConsts &c, Agent &a, u
         if (t_self >= j && t_self < end) near &= ~(1u << (end - 1u - t_self));
         uint32_t next = end;
         while (near) {
-            // Each lane first passes over its near candidates that fail the
-            // contact filter (they change nothing) up to the next that
-            // passes it; then the lanes that found one run the contact
-            // response together, so a wave runs it about as often as its
-            // busiest lane has contacts rather than once for every candidate
-            // position at which some lane has one.
-            uint32_t t, l;
+            uint32_t t, l = 0;
             float4 bp;
-            bool hit;
-            do {
+            bool moved;
+            if (SPRING) {
                 const uint32_t lz = (uint32_t)__builtin_clz(near);
                 t = end - 32u + lz;
                 near ^= 0x80000000u >> lz;
                 if (FROM_LDS) {
-                    l = lds_of(t);
-                    bp = s_pos[l];
+                    bp = s_pos[lds_of(t)];
                 } else {
                     l = ld_sc<SC>(&perm[slot_of(t)]);
                     bp = pos_i[l];
                 }
-                hit = contact_filter(a, bp);
-            } while (!hit && near);
-            if (!hit) break;
-            bool moved;
-            uint2 q = make_uint2(0u, 0u);
-            // the neighbour's velocity and mass, read before the contact's
-            // square root and quotients so that its latency overlaps them
-            if (!SPRING) q = vel_mass_of(vfme_i[FROM_LDS ? s_src[l] : l]);
-            moved = collide_candidate<SPRING>(c, a, bp, [&]() { return q; });
+                moved = collide_candidate<SPRING>(c, a, bp, []() { return make_uint2(0u, 0u); });
+            } else {
+                // Rigid response: each lane first passes over its near
+                // candidates that fail the contact filter (they change
+                // nothing) up to the next that passes it; then the lanes that
+                // found one run the contact response together, so a wave
+                // runs it about as often as its busiest lane has contacts
+                // rather than once for every candidate position at which some
+                // lane has one.  (Spring scenes, whose crowded lanes have
+                // contacts at most candidates, lose by it.)
+                bool hit;
+                do {
+                    const uint32_t lz = (uint32_t)__builtin_clz(near);
+                    t = end - 32u + lz;
+                    near ^= 0x80000000u >> lz;
+                    if (FROM_LDS) {
+                        l = lds_of(t);
+                        bp = s_pos[l];
+                    } else {
+                        l = ld_sc<SC>(&perm[slot_of(t)]);
+                        bp = pos_i[l];
+                    }
+                    hit = contact_filter(a, bp);
+                } while (!hit && near);
+                if (!hit) break;
+                // the neighbour's velocity and mass, read before the
+                // contact's square root and quotients so that its latency
+                // overlaps them
+                const uint2 q = vel_mass_of(vfme_i[FROM_LDS ? s_src[l] : l]);
+                moved = collide_candidate<SPRING>(c, a, bp, [&]() { return q; });
+            }
             if (moved) {
                 const float ex = a.px - rx, ey = a.py - ry, ez = a.pz - rz;
                 if (!(ex * ex + ey * ey + ez * ez <= still)) {
