@@ -1872,7 +1872,8 @@ __device__ __forceinline__ void physics_step_body(
                                  : 0u;
             }
 #pragma unroll
-            for (uint32_t i = 0; i < kBatch; ++i) bp[i] = pos_i[bo[i]];  // slot 0's beyond wtot
+            for (uint32_t i = 0; i < kBatch; ++i)
+                bp[i] = pos_i[bo[i]];  // past wtot: input 0's position, not stored
             // all of them in flight before the first is used (keeps the
             // compiler from moving each read into its store's branch)
 #pragma unroll
