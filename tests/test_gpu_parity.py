@@ -731,6 +731,26 @@ def test_every_tuning_matches_the_oracle(gpu, oracle, scene):
             assert np.array_equal(g.debug()["next_keys"], kr), tuning
 
 
+@pytest.mark.parametrize("spring", [False, True])
+@pytest.mark.parametrize("tuning", [(1.0, 896, 512, 0), (1.0, 1408, 512, 0),
+                                    (0.0625, 1408, 512, 1)])
+def test_window_staging_batches(gpu, oracle, spring, tuning):
+    """A collide block stages its windows four slots per thread at a time
+    (1024 slots a batch).  Clumps of 32,768 spheres on a 64 x 64 grid give
+    blocks whose windows hold from ~800 to ~1400 spheres: with the 896-slot
+    pool some overflow it and read from global memory; with the 1408-slot
+    pool some need a second, partly used batch.  Rigid and spring responses
+    (the rigid blocks also stage each slot's input index)."""
+    cfg = replace(CONFIGS["spring_clumps_1m"], number_of_agents=32_768, grids=64, clumps=4,
+                  enable_spring=spring)
+    print(f"[staging] spring={spring} {tuning}", flush=True)
+    g, r = make_pair(oracle, cfg, seed=3)
+    with g:
+        g.set_tuning(*tuning)
+        compare_frames(g, r, 12, check_every=4)
+    r.close()
+
+
 @pytest.mark.parametrize("dense", [0, 1])
 @pytest.mark.parametrize("pool", [896, 1408])
 def test_large_scene_collide_variant_bit_exact(gpu, oracle, pool, dense):
